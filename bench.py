@@ -1,0 +1,166 @@
+"""Benchmark: LocVolCalib options calibrated per second on one MI355X.
+
+Workload (the plain method at 256x256x64): main(128, 256, 256, 64, s0=0.03,
+t=5, alpha=0.2, nu=0.6, beta=0.5) -> 128 prices.  One "step" is one call of
+the entry point over the 128 options (setup, 63 ADI time steps, price
+gather), inputs resident on the device (the entry takes scalars only), the
+result left in device memory.
+
+Reported:
+  value       options/s of the FP32 + FP64-residual-refinement form
+              (futhark_entry_main, the headline form);
+  fp64        options/s of the FP64 form (futhark_entry_main_f64) and the
+              ratio; p50/p99 per-call latency of both;
+  roofline    the dominant kernel (the x sweep of the FP32+refine form):
+              algorithmic bytes per launch (read R + write Y, 2*N*4 bytes per
+              option, 128 options) / its mean duration from HIP events
+              recorded around every launch on the library's stream during the
+              timed calls, against 8 TB/s;
+  accuracy    max relative price difference FP32+refine vs FP64;
+  cpu_baseline  the oracle (C restatement of LocVolCalib.fut, scan-form
+              tridiagonal solve as the reference) on the host cores, OpenMP
+              over options, on a time-bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "futhark-benchmarks_amd"))
+sys.path.insert(0, ROOT)
+
+OUTER, NX, NY, NT = 128, 256, 256, 64
+PARAMS = (0.03, 5.0, 0.2, 0.6, 0.5)
+HBM_PEAK_GBS = 8000.0
+
+
+def time_calls(ctx, raw, free, steps, warmup):
+    for _ in range(warmup):
+        free(raw(OUTER, NX, NY, NT, *PARAMS))
+    ctx.sync()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        h = raw(OUTER, NX, NY, NT, *PARAMS)
+        ctx.sync()
+        lat.append(time.perf_counter() - t0)
+        free(h)
+    ctx.sync()
+    total = time.perf_counter() - t_start
+    return total, lat
+
+
+def pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v) * 1e3, q))
+
+
+def cpu_baseline(budget_s=12.0):
+    import numpy as np
+    from oracle import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    chunk = 2 * threads
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        idx = np.arange(done, done + chunk) % OUTER
+        K = (np.float32(0.001) * idx.astype(np.float32)).astype(np.float32)
+        n = len(idx)
+        oracle.batch([NX] * n, [NY] * n, [NT] * n, [PARAMS[0]] * n, K, [PARAMS[1]] * n,
+                     [PARAMS[2]] * n, [PARAMS[3]] * n, [PARAMS[4]] * n,
+                     dtype=np.float32, thomas=False, nthreads=threads)
+        done += n
+    el = time.perf_counter() - t0
+    return {"value": round(done / el, 3), "unit": "options/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{done} options of main({OUTER},{NX},{NY},{NT}) (f32, scan-form "
+                      f"tridagPar as LocVolCalib.fut:68-124), OpenMP over options, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(options shard across GPUs with no exchange); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import locvolcalib as lvc
+
+    with lvc.LocVolCalib(device=0, profiling=True) as ctx:
+        # FP32 + refinement (headline)
+        ctx.pause_profiling()
+        total32, lat32 = None, None
+        for _ in range(args.warmup):
+            ctx.free_f32(ctx.main_raw(OUTER, NX, NY, NT, *PARAMS))
+        ctx.sync()
+        ctx.report()  # drain
+        ctx.unpause_profiling()
+        total32, lat32 = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps, 0)
+        rep32 = json.loads(ctx.report())
+        ctx.pause_profiling()
+        # FP64
+        total64, lat64 = time_calls(ctx, ctx.main_f64_raw, ctx.free_f64, args.steps,
+                                    args.warmup)
+        p32 = ctx.main(OUTER, NX, NY, NT, *PARAMS).astype(np.float64)
+        p64 = ctx.main_f64(OUTER, NX, NY, NT, *PARAMS)
+
+    v32 = OUTER * args.steps / total32
+    v64 = OUTER * args.steps / total64
+    xs = rep32.get("lvc_xsweep", {"count": 0, "total_us": 0.0})
+    ys = rep32.get("lvc_ysweep", {"count": 0, "total_us": 0.0})
+    dom, other = (xs, ys) if xs["total_us"] >= ys["total_us"] else (ys, xs)
+    dom_name = "lvc_xsweep" if dom is xs else "lvc_ysweep"
+    avg_us = dom["total_us"] / max(dom["count"], 1)
+    bytes_launch = OUTER * 2 * NX * NY * 4
+    achieved = bytes_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    rel = float(np.max(np.abs(p32 - p64) / np.abs(p64)))
+
+    out = {
+        "metric": "options calibrated/sec @256x256x64 FP32+refine vs FP64 plain; p50/p99 latency",
+        "value": round(v32, 2),
+        "unit": "options/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total32 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "LocVolCalib main(128, 256, 256, 64) plain method, "
+                               "s0=0.03 t=5 alpha=0.2 nu=0.6 beta=0.5",
+                   "outer": OUTER, "numX": NX, "numY": NY, "numT": NT,
+                   "precision": "FP32 state and sweeps + one FP64-residual refinement "
+                                "per implicit solve; FP64 setup and price gather"},
+        "p50_ms": round(pct(lat32, 50), 4),
+        "p99_ms": round(pct(lat32, 99), 4),
+        "fp64": {"value": round(v64, 2), "unit": "options/s",
+                 "ms_per_step": round(1e3 * total64 / args.steps, 4),
+                 "p50_ms": round(pct(lat64, 50), 4), "p99_ms": round(pct(lat64, 99), 4)},
+        "ratio_fp32refine_vs_fp64": round(v32 / v64, 3),
+        "max_rel_err_fp32refine_vs_fp64": rel,
+        "kernels_us_per_launch": {k: round(v["total_us"] / max(v["count"], 1), 3)
+                                  for k, v in rep32.items()},
+        "roofline": {"bound": "hbm", "kernel": dom_name,
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,413 @@
+/*
+ * Entry points of the LocVolCalib library and the batch engine behind them.
+ *
+ * A batch is a set of independent problems (LocVolCalib.fut:211-236,
+ * `value`), each with its own sizes and parameters.  The reference's `main`
+ * (:238-252) is the batch of `outer` problems sharing everything but the
+ * strike 0.001*i.  Per time step the engine launches the x sweep and the y
+ * sweep once for the whole batch; a problem whose numT-1 steps are done
+ * drops out (its workgroups return at once) until the longest has finished.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "lvc_runtime.h"
+
+void *lvc_array_mem_f64(const struct futhark_f64_1d *a);
+void *lvc_array_mem_i32(const struct futhark_i32_1d *a);
+int64_t lvc_array_len_f64(const struct futhark_f64_1d *a);
+int64_t lvc_array_len_i32(const struct futhark_i32_1d *a);
+struct futhark_f32_1d *lvc_array_adopt_f32(void *mem, int64_t n);
+struct futhark_f64_1d *lvc_array_adopt_f64(void *mem, int64_t n);
+
+int lvc_validate_sizes(struct futhark_context *ctx, int64_t nx, int64_t ny,
+                       int64_t nt) {
+  /* LocVolCalib.fut:37 indexes x[i-1], x[i+1] for i in 1...n-2, and the
+   * timeline needs two points (:217). */
+  if (nx < 3 || ny < 3) {
+    lvc_set_error(ctx, "numX and numY must be at least 3 (got %lld, %lld)",
+                  (long long)nx, (long long)ny);
+    return 1;
+  }
+  if (nx > LVC_MAX_N || ny > LVC_MAX_N) {
+    lvc_set_error(ctx, "numX and numY must be at most %d (got %lld, %lld)",
+                  LVC_MAX_N, (long long)nx, (long long)ny);
+    return 1;
+  }
+  if (nt < 2) {
+    lvc_set_error(ctx, "numT must be at least 2 (got %lld)", (long long)nt);
+    return 1;
+  }
+  return 0;
+}
+
+/* Smallest supported segment length >= want whose lane count fits a
+ * wavefront (<= 64 lanes per system). */
+static int fit_seg(int n, int want) {
+  static const int segs[] = {4, 8, 16};
+  for (int i = 0; i < 3; i++)
+    if (segs[i] >= want && lvc_lanes_for(n, segs[i]) <= 64)
+      return segs[i];
+  return 16;
+}
+
+void lvc_choose_segs(struct futhark_context *ctx, int precision, int nx, int ny,
+                     int *seg_x, int *seg_y) {
+  int sx = ctx->cfg ? ctx->cfg->seg_x : 0, sy = ctx->cfg ? ctx->cfg->seg_y : 0;
+  int tx = 0, ty = 0;
+  if (!sx || !sy)
+    lvc_tune_pick(ctx, precision, nx, ny, &tx, &ty);
+  *seg_x = fit_seg(nx, sx ? sx : tx);
+  *seg_y = fit_seg(ny, sy ? sy : ty);
+}
+
+static int64_t align64(int64_t v) { return (v + 63) & ~(int64_t)63; }
+
+int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
+                    const lvc_problem *probs, int precision, int seg_x,
+                    int seg_y) {
+  memset(b, 0, sizeof(*b));
+  b->n = n;
+  b->precision = precision;
+  b->seg_x = seg_x;
+  b->seg_y = seg_y;
+  if (n == 0)
+    return 0;
+  const size_t real = precision == LVC_FP64 ? sizeof(double) : sizeof(float);
+  b->h_probs = malloc((size_t)n * sizeof(lvc_problem));
+  if (!b->h_probs) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  int64_t geo = 0, state = 0, nxt = 0, nyt = 0, steps = 0;
+  for (int p = 0; p < n; p++) {
+    lvc_problem q = probs[p];
+    const int gx = lvc_lanes_for(q.nx, seg_x), gy = lvc_lanes_for(q.ny, seg_y);
+    const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
+    q.geo_off = geo;
+    q.state_off = state;
+    q.k = -1;
+    q.status = 0;
+    q.ydone = 0;
+    q.ytiles = (q.nx + cols - 1) / cols;
+    geo += align64(LVC_GEO_LEN(q.nx, q.ny));
+    state += align64((int64_t)q.nx * q.ny);
+    nxt += (q.ny + rows - 1) / rows;
+    nyt += q.ytiles;
+    if (q.nt - 1 > steps)
+      steps = q.nt - 1;
+    b->h_probs[p] = q;
+  }
+  if (nxt + nyt > (int64_t)1 << 30) {
+    lvc_set_error(ctx, "batch too large");
+    return 1;
+  }
+  b->nxt = (int)nxt;
+  b->nyt = (int)nyt;
+  b->steps = steps;
+  lvc_tile *tiles = malloc((size_t)(nxt + nyt) * sizeof(lvc_tile));
+  if (!tiles) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  int64_t tx = 0, ty = nxt;
+  for (int p = 0; p < n; p++) {
+    const lvc_problem *q = &b->h_probs[p];
+    const int gx = lvc_lanes_for(q->nx, seg_x), gy = lvc_lanes_for(q->ny, seg_y);
+    const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
+    for (int r0 = 0; r0 < q->ny; r0 += rows) {
+      lvc_tile t = {p, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx};
+      tiles[tx++] = t;
+    }
+    for (int c0 = 0; c0 < q->nx; c0 += cols) {
+      lvc_tile t = {p, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy};
+      tiles[ty++] = t;
+    }
+  }
+  b->probs_bytes = (size_t)n * sizeof(lvc_problem);
+  b->geo_bytes = (size_t)geo * sizeof(double);
+  b->state_bytes = (size_t)state * real;
+  b->tiles_bytes = (size_t)(nxt + nyt) * sizeof(lvc_tile);
+  int rc = lvc_dev_alloc(ctx, b->probs_bytes, (void **)&b->d_probs) ||
+           lvc_dev_alloc(ctx, b->geo_bytes, (void **)&b->d_geo) ||
+           lvc_dev_alloc(ctx, b->state_bytes, &b->d_R) ||
+           lvc_dev_alloc(ctx, b->state_bytes, &b->d_Y) ||
+           lvc_dev_alloc(ctx, b->tiles_bytes, (void **)&b->d_tiles);
+  if (!rc)
+    rc = lvc_upload(ctx, b->d_probs, b->h_probs, b->probs_bytes) ||
+         lvc_upload(ctx, b->d_tiles, tiles, b->tiles_bytes);
+  free(tiles);
+  return rc;
+}
+
+void lvc_batch_release(struct futhark_context *ctx, lvc_batch *b) {
+  lvc_dev_free(ctx, b->d_probs, b->probs_bytes);
+  lvc_dev_free(ctx, b->d_geo, b->geo_bytes);
+  lvc_dev_free(ctx, b->d_R, b->state_bytes);
+  lvc_dev_free(ctx, b->d_Y, b->state_bytes);
+  lvc_dev_free(ctx, b->d_tiles, b->tiles_bytes);
+  free(b->h_probs);
+  memset(b, 0, sizeof(*b));
+}
+
+int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
+  if (b->n == 0)
+    return 0;
+  void *st = (void *)ctx->stream;
+  int tok = lvc_prof_begin(ctx, "lvc_setup");
+  int e = lvc_launch_setup(st, b->precision, b->d_probs, b->n, b->d_geo, b->d_R);
+  lvc_prof_end(ctx, tok);
+  if (lvc_check_hip(ctx, e, "setup kernel"))
+    return 1;
+  const lvc_tile *xt = b->d_tiles, *yt = b->d_tiles + b->nxt;
+  for (int64_t s = 0; s < b->steps; s++) {
+    tok = lvc_prof_begin(ctx, "lvc_xsweep");
+    e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt, b->nxt,
+                          b->d_geo, b->d_R, b->d_Y);
+    lvc_prof_end(ctx, tok);
+    if (lvc_check_hip(ctx, e, "x sweep kernel"))
+      return 1;
+    tok = lvc_prof_begin(ctx, "lvc_ysweep");
+    e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt, b->nyt,
+                          b->d_geo, b->d_Y, b->d_R);
+    lvc_prof_end(ctx, tok);
+    if (lvc_check_hip(ctx, e, "y sweep kernel"))
+      return 1;
+  }
+  return 0;
+}
+
+int lvc_batch_gather(struct futhark_context *ctx, lvc_batch *b, void *out,
+                     int out_f64) {
+  if (b->n == 0)
+    return 0;
+  int tok = lvc_prof_begin(ctx, "lvc_gather");
+  int e = lvc_launch_gather((void *)ctx->stream, b->precision, b->d_probs, b->n,
+                            b->d_R, out, out_f64);
+  lvc_prof_end(ctx, tok);
+  return lvc_check_hip(ctx, e, "gather kernel");
+}
+
+/* Runs a batch end to end into a new device array of n prices. */
+static int run_batch(struct futhark_context *ctx, int n, const lvc_problem *probs,
+                     int precision, int out_f64, void **out_mem) {
+  int nxm = 3, nym = 3;
+  for (int p = 0; p < n; p++) {
+    if (probs[p].nx > nxm)
+      nxm = probs[p].nx;
+    if (probs[p].ny > nym)
+      nym = probs[p].ny;
+  }
+  int sx, sy;
+  lvc_choose_segs(ctx, precision, nxm, nym, &sx, &sy);
+  lvc_batch b;
+  const size_t ob = (size_t)n * (out_f64 ? sizeof(double) : sizeof(float));
+  if (lvc_dev_alloc(ctx, ob, out_mem))
+    return 1;
+  int rc = lvc_batch_build(ctx, &b, n, probs, precision, sx, sy);
+  if (!rc)
+    rc = lvc_batch_run(ctx, &b);
+  if (!rc)
+    rc = lvc_batch_gather(ctx, &b, *out_mem, out_f64);
+  lvc_batch_release(ctx, &b);
+  if (rc) {
+    lvc_dev_free(ctx, *out_mem, ob);
+    *out_mem = NULL;
+  }
+  return rc;
+}
+
+static void fill_main(lvc_problem *probs, int outer, int nx, int ny, int nt,
+                      double s0, double t, double alpha, double nu, double beta) {
+  memset(probs, 0, (size_t)outer * sizeof(*probs));
+  for (int i = 0; i < outer; i++) {
+    probs[i].nx = nx;
+    probs[i].ny = ny;
+    probs[i].nt = nt;
+    probs[i].s0 = s0;
+    probs[i].t = t;
+    probs[i].alpha = alpha;
+    probs[i].nu = nu;
+    probs[i].beta = beta;
+  }
+}
+
+/* LocVolCalib.fut:238-252 */
+int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0,
+                       const int32_t outer, const int32_t numX,
+                       const int32_t numY, const int32_t numT, const float s0,
+                       const float t, const float alpha, const float nu,
+                       const float beta) {
+  if (!out0 || !ctx)
+    return 1;
+  *out0 = NULL;
+  if (outer < 0) {
+    lvc_set_error(ctx, "outer_loop_count must be non-negative (got %d)", outer);
+    return 1;
+  }
+  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx))
+    return 1;
+  lvc_problem *probs = malloc((size_t)(outer ? outer : 1) * sizeof(*probs));
+  if (!probs) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  fill_main(probs, outer, numX, numY, numT, s0, t, alpha, nu, beta);
+  for (int i = 0; i < outer; i++)
+    probs[i].strike = (double)(0.001f * (float)(int64_t)i); /* :247, in f32 */
+  pthread_mutex_lock(&ctx->lock);
+  void *mem = NULL;
+  int rc = run_batch(ctx, outer, probs, ctx->cfg ? ctx->cfg->precision
+                                                 : LVC_FP32_REFINE, 0, &mem);
+  pthread_mutex_unlock(&ctx->lock);
+  free(probs);
+  if (rc)
+    return 1;
+  *out0 = lvc_array_adopt_f32(mem, outer);
+  if (!*out0) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  return 0;
+}
+
+int futhark_entry_main_f64(struct futhark_context *ctx,
+                           struct futhark_f64_1d **out0, const int32_t outer,
+                           const int32_t numX, const int32_t numY,
+                           const int32_t numT, const double s0, const double t,
+                           const double alpha, const double nu,
+                           const double beta) {
+  if (!out0 || !ctx)
+    return 1;
+  *out0 = NULL;
+  if (outer < 0) {
+    lvc_set_error(ctx, "outer_loop_count must be non-negative (got %d)", outer);
+    return 1;
+  }
+  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx))
+    return 1;
+  lvc_problem *probs = malloc((size_t)(outer ? outer : 1) * sizeof(*probs));
+  if (!probs) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  fill_main(probs, outer, numX, numY, numT, s0, t, alpha, nu, beta);
+  for (int i = 0; i < outer; i++)
+    probs[i].strike = 0.001 * (double)(int64_t)i;
+  pthread_mutex_lock(&ctx->lock);
+  void *mem = NULL;
+  int rc = run_batch(ctx, outer, probs, LVC_FP64, 1, &mem);
+  pthread_mutex_unlock(&ctx->lock);
+  free(probs);
+  if (rc)
+    return 1;
+  *out0 = lvc_array_adopt_f64(mem, outer);
+  if (!*out0) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  return 0;
+}
+
+static int download(struct futhark_context *ctx, void *dst, const void *src,
+                    size_t bytes) {
+  if (bytes == 0)
+    return 0;
+  if (lvc_check_hip(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                                        ctx->stream),
+                    "hipMemcpyAsync"))
+    return 1;
+  return lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream),
+                       "hipStreamSynchronize");
+}
+
+int futhark_entry_main_batch(struct futhark_context *ctx,
+                             struct futhark_f64_1d **out0,
+                             const struct futhark_i32_1d *numX,
+                             const struct futhark_i32_1d *numY,
+                             const struct futhark_i32_1d *numT,
+                             const struct futhark_f64_1d *s0,
+                             const struct futhark_f64_1d *strike,
+                             const struct futhark_f64_1d *t,
+                             const struct futhark_f64_1d *alpha,
+                             const struct futhark_f64_1d *nu,
+                             const struct futhark_f64_1d *beta,
+                             int32_t precision) {
+  if (!out0)
+    return 1;
+  *out0 = NULL;
+  if (!lvc_ready(ctx))
+    return 1;
+  if (!numX || !numY || !numT || !s0 || !strike || !t || !alpha || !nu || !beta) {
+    lvc_set_error(ctx, "NULL input array");
+    return 1;
+  }
+  if (precision < LVC_FP32 || precision > LVC_FP64) {
+    lvc_set_error(ctx, "unknown precision %d", precision);
+    return 1;
+  }
+  const int64_t n = lvc_array_len_i32(numX);
+  const struct futhark_f64_1d *reals[6] = {s0, strike, t, alpha, nu, beta};
+  if (lvc_array_len_i32(numY) != n || lvc_array_len_i32(numT) != n) {
+    lvc_set_error(ctx, "input arrays differ in length");
+    return 1;
+  }
+  for (int k = 0; k < 6; k++)
+    if (lvc_array_len_f64(reals[k]) != n) {
+      lvc_set_error(ctx, "input arrays differ in length");
+      return 1;
+    }
+  if (n > (1 << 24)) {
+    lvc_set_error(ctx, "too many problems (%lld)", (long long)n);
+    return 1;
+  }
+  int32_t *iv = malloc((size_t)(n ? n : 1) * 3 * sizeof(int32_t));
+  double *rv = malloc((size_t)(n ? n : 1) * 6 * sizeof(double));
+  lvc_problem *probs = calloc((size_t)(n ? n : 1), sizeof(*probs));
+  if (!iv || !rv || !probs) {
+    free(iv);
+    free(rv);
+    free(probs);
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  pthread_mutex_lock(&ctx->lock);
+  int rc = 0;
+  rc = rc || download(ctx, iv, lvc_array_mem_i32(numX), (size_t)n * 4);
+  rc = rc || download(ctx, iv + n, lvc_array_mem_i32(numY), (size_t)n * 4);
+  rc = rc || download(ctx, iv + 2 * n, lvc_array_mem_i32(numT), (size_t)n * 4);
+  for (int k = 0; k < 6 && !rc; k++)
+    rc = download(ctx, rv + k * n, lvc_array_mem_f64(reals[k]), (size_t)n * 8);
+  for (int64_t p = 0; p < n && !rc; p++) {
+    probs[p].nx = iv[p];
+    probs[p].ny = iv[n + p];
+    probs[p].nt = iv[2 * n + p];
+    probs[p].s0 = rv[p];
+    probs[p].strike = rv[n + p];
+    probs[p].t = rv[2 * n + p];
+    probs[p].alpha = rv[3 * n + p];
+    probs[p].nu = rv[4 * n + p];
+    probs[p].beta = rv[5 * n + p];
+    if (lvc_validate_sizes(ctx, probs[p].nx, probs[p].ny, probs[p].nt)) {
+      char *inner = futhark_context_get_error(ctx);
+      lvc_set_error(ctx, "problem %lld: %s", (long long)p, inner ? inner : "");
+      free(inner);
+      rc = 1;
+    }
+  }
+  void *mem = NULL;
+  if (!rc)
+    rc = run_batch(ctx, (int)n, probs, precision, 1, &mem);
+  pthread_mutex_unlock(&ctx->lock);
+  free(iv);
+  free(rv);
+  free(probs);
+  if (rc)
+    return 1;
+  *out0 = lvc_array_adopt_f64(mem, n);
+  if (!*out0) {
+    lvc_set_error(ctx, "host allocation failed");
+    return 1;
+  }
+  return 0;
+}

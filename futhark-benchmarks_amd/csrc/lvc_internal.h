@@ -1,0 +1,87 @@
+/*
+ * Internal interface between the C host code and the HIP kernels: device
+ * data layouts and the launcher functions (plain C linkage, HIP types only
+ * as opaque stream handles).
+ *
+ * Device data of one set of problems (a "batch"):
+ *   - lvc_problem[n]   descriptors (sizes, parameters, offsets, step index);
+ *   - geometry (f64)   per problem, at geo_off, LVC_GEO_LEN(nx, ny) doubles:
+ *       x[nx] | ax[nx] | cx[nx] | y[ny] | ay[ny] | cy[ny]
+ *     where ax/cx are -0.25 * x^(2 beta) * dxx0/dxx2 (the x operator's off-
+ *     diagonal taps without the time factor exp(2y - nu^2 t)) and ay/cy are
+ *     -0.25 * nu^2 * dyy0/dyy2 (the timeline t*k/(nt-1) is computed where it
+ *     is used);
+ *   - state R (real)   per problem, at state_off, [ny][nx] row-major;
+ *   - state Y (real)   same layout, the x-sweep's output.
+ * "real" is float or double according to the precision of the batch.
+ */
+#ifndef LVC_INTERNAL_H
+#define LVC_INTERNAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum lvc_precision {
+  LVC_FP32 = 0,        /* stencils and sweeps in FP32 */
+  LVC_FP32_REFINE = 1, /* FP32 + one FP64-residual refinement per solve */
+  LVC_FP64 = 2         /* everything FP64 */
+};
+
+typedef struct lvc_problem {
+  int32_t nx, ny, nt;
+  int32_t xindex, yindex;  /* written by the setup kernel */
+  int32_t k;               /* next time index to process; < 0 when done */
+  int32_t status;          /* 0 = ok, 3 = price index out of range */
+  int32_t ytiles;          /* y-sweep workgroups of this problem per step */
+  int32_t ydone;           /* y-sweep workgroups finished in this step */
+  int64_t state_off;       /* element offset of R and Y */
+  int64_t geo_off;         /* element offset into the f64 geometry buffer */
+  double s0, strike, t, alpha, nu, beta;
+} lvc_problem;
+
+/* One workgroup's share of a sweep: `count` rows (x sweep) or columns (y
+ * sweep) of problem `problem`, starting at `first`, with `lanes` lanes (a
+ * power of two) per row/column. */
+typedef struct lvc_tile {
+  int32_t problem;
+  int32_t first;
+  int32_t count;
+  int32_t lanes;
+} lvc_tile;
+
+#define LVC_GEO_LEN(nx, ny) (3 * (int64_t)(nx) + 3 * (int64_t)(ny))
+#define LVC_BLOCK 256 /* threads per workgroup of the sweeps */
+#define LVC_MAX_N 1024 /* largest numX / numY accepted (64 lanes x 16) */
+
+/* Supported segment lengths (elements of one system per lane). */
+int lvc_seg_supported(int seg);
+
+/* Launchers: return 0 or a hipError_t value.  `stream` is a hipStream_t. */
+int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
+                     double *geo, void *R);
+int lvc_launch_xsweep(void *stream, int precision, int seg,
+                      const lvc_problem *probs, const lvc_tile *tiles,
+                      int ntiles, const double *geo, const void *R, void *Y);
+int lvc_launch_ysweep(void *stream, int precision, int seg,
+                      const lvc_problem *probs, const lvc_tile *tiles,
+                      int ntiles, const double *geo, const void *Y, void *R);
+int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
+                      int nprob, const void *R, void *out, int out_f64);
+
+/* Lanes per row/column and rows/columns per workgroup for a system of
+ * length n split in segments of `seg`. */
+static inline int lvc_lanes_for(int n, int seg) {
+  int P = (n + seg - 1) / seg, G = 1;
+  while (G < P)
+    G <<= 1;
+  return G;
+}
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LVC_INTERNAL_H */

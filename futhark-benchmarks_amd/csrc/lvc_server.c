@@ -1,0 +1,397 @@
+/*
+ * Serving form: continuous batching of independent calibration requests
+ * (each one LocVolCalib.fut:211-236 `value` with its own sizes and
+ * parameters).  No counterpart in the reference.
+ *
+ * The server owns `max_slots` fixed device slots.  A slot holds one resident
+ * request: its descriptor, its geometry and its two state grids (R, Y), all
+ * sized for the largest accepted grid, so admission and completion never
+ * move another request's data.  One engine tick (lvc_server_step):
+ *   1. admits queued requests, in arrival order, while a slot is free and the
+ *      resident bytes (the sum of the admitted requests' own grid sizes) stay
+ *      within the memory budget; each admission uploads its descriptor and
+ *      runs the setup kernel for that slot;
+ *   2. launches the x sweep and the y sweep once over every resident request
+ *      (each advances one time step);
+ *   3. gathers the price of every request that has just done its last step,
+ *      copies those prices back and frees their slots.
+ * Every kernel runs on the context's stream, so the slot reuse in step 1 is
+ * ordered after the completed request's last use.  A request's arithmetic
+ * depends only on its own inputs and the server's fixed segment lengths,
+ * never on which other requests are resident.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "lvc_runtime.h"
+
+#define SLOT_MAX_N 512 /* largest numX / numY a server slot holds */
+
+typedef struct {
+  int64_t id;
+  struct lvc_request req;
+} queued_req;
+
+typedef struct {
+  int state; /* 0 queued, 1 resident, 2 done */
+  double price;
+} result_rec;
+
+struct lvc_server {
+  struct futhark_context *ctx;
+  int max_slots, precision, seg_x, seg_y;
+  int64_t budget, resident_bytes;
+  size_t real;
+  int64_t slot_elems, slot_geo;
+  /* slots */
+  lvc_problem *h_slots;
+  int64_t *slot_id;  /* request id, -1 when free */
+  int *steps_left;
+  int nresident;
+  lvc_problem *d_slots;
+  double *d_geo;
+  void *d_R, *d_Y;
+  double *d_out;
+  double *h_out;
+  /* tiles of the resident set */
+  lvc_tile *h_tiles, *d_tiles;
+  int nxt, nyt, tiles_cap, tiles_dirty;
+  /* queue and results */
+  queued_req *queue;
+  int64_t qhead, qtail, qcap;
+  result_rec *results;
+  int64_t nresults, rcap;
+  int64_t completed;
+};
+
+static size_t request_bytes(const struct lvc_server *s, const struct lvc_request *r) {
+  return 2 * (size_t)r->numX * (size_t)r->numY * s->real +
+         (size_t)LVC_GEO_LEN(r->numX, r->numY) * sizeof(double);
+}
+
+struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots,
+                                  int64_t mem_budget_bytes, int32_t precision) {
+  if (!lvc_ready(ctx))
+    return NULL;
+  if (max_slots <= 0 || max_slots > (1 << 20) || precision < LVC_FP32 ||
+      precision > LVC_FP64 || mem_budget_bytes < 0) {
+    lvc_set_error(ctx, "lvc_server_new: invalid arguments");
+    return NULL;
+  }
+  struct lvc_server *s = calloc(1, sizeof(*s));
+  if (!s)
+    return NULL;
+  s->ctx = ctx;
+  s->max_slots = max_slots;
+  s->precision = precision;
+  s->budget = mem_budget_bytes;
+  s->real = precision == LVC_FP64 ? sizeof(double) : sizeof(float);
+  s->slot_elems = (int64_t)SLOT_MAX_N * SLOT_MAX_N;
+  s->slot_geo = LVC_GEO_LEN(SLOT_MAX_N, SLOT_MAX_N);
+  pthread_mutex_lock(&ctx->lock);
+  /* segment lengths fixed for the server's life (one kernel variant for the
+   * whole resident set); tuned for the largest slot grid */
+  lvc_choose_segs(ctx, precision, 256, 256, &s->seg_x, &s->seg_y);
+  if (lvc_lanes_for(SLOT_MAX_N, s->seg_x) > 64)
+    s->seg_x = 8;
+  if (lvc_lanes_for(SLOT_MAX_N, s->seg_y) > 64)
+    s->seg_y = 8;
+  s->h_slots = calloc((size_t)max_slots, sizeof(lvc_problem));
+  s->slot_id = malloc((size_t)max_slots * sizeof(int64_t));
+  s->steps_left = calloc((size_t)max_slots, sizeof(int));
+  s->h_out = calloc((size_t)max_slots, sizeof(double));
+  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_out;
+  if (rc)
+    lvc_set_error(ctx, "host allocation failed");
+  for (int i = 0; !rc && i < max_slots; i++) {
+    s->slot_id[i] = -1;
+    s->h_slots[i].k = -1;
+  }
+  if (!rc)
+    rc = lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(lvc_problem), (void **)&s->d_slots) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_geo * sizeof(double), (void **)&s->d_geo) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_elems * s->real, &s->d_R) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_elems * s->real, &s->d_Y) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(double), (void **)&s->d_out);
+  if (!rc)
+    rc = lvc_upload(ctx, s->d_slots, s->h_slots, (size_t)max_slots * sizeof(lvc_problem));
+  pthread_mutex_unlock(&ctx->lock);
+  if (rc) {
+    lvc_server_free(s);
+    return NULL;
+  }
+  return s;
+}
+
+void lvc_server_free(struct lvc_server *s) {
+  if (!s)
+    return;
+  struct futhark_context *ctx = s->ctx;
+  pthread_mutex_lock(&ctx->lock);
+  (void)hipStreamSynchronize(ctx->stream);
+  lvc_dev_free(ctx, s->d_slots, (size_t)s->max_slots * sizeof(lvc_problem));
+  lvc_dev_free(ctx, s->d_geo, (size_t)s->max_slots * s->slot_geo * sizeof(double));
+  lvc_dev_free(ctx, s->d_R, (size_t)s->max_slots * s->slot_elems * s->real);
+  lvc_dev_free(ctx, s->d_Y, (size_t)s->max_slots * s->slot_elems * s->real);
+  lvc_dev_free(ctx, s->d_out, (size_t)s->max_slots * sizeof(double));
+  if (s->d_tiles)
+    lvc_dev_free(ctx, s->d_tiles, (size_t)s->tiles_cap * sizeof(lvc_tile));
+  pthread_mutex_unlock(&ctx->lock);
+  free(s->h_slots);
+  free(s->slot_id);
+  free(s->steps_left);
+  free(s->h_out);
+  free(s->h_tiles);
+  free(s->queue);
+  free(s->results);
+  free(s);
+}
+
+int64_t lvc_server_submit(struct lvc_server *s, const struct lvc_request *r) {
+  if (!s || !r)
+    return -1;
+  struct futhark_context *ctx = s->ctx;
+  if (lvc_validate_sizes(ctx, r->numX, r->numY, r->numT))
+    return -1;
+  if (r->numX > SLOT_MAX_N || r->numY > SLOT_MAX_N) {
+    lvc_set_error(ctx, "server slots hold grids up to %d x %d", SLOT_MAX_N,
+                  SLOT_MAX_N);
+    return -1;
+  }
+  if (lvc_lanes_for(r->numX, s->seg_x) > 64 || lvc_lanes_for(r->numY, s->seg_y) > 64) {
+    lvc_set_error(ctx, "grid too large for the server's segment lengths");
+    return -1;
+  }
+  if (s->budget > 0 && (int64_t)request_bytes(s, r) > s->budget) {
+    lvc_set_error(ctx, "request needs %zu bytes, more than the budget of %lld",
+                  request_bytes(s, r), (long long)s->budget);
+    return -1;
+  }
+  if (s->qtail == s->qcap) {
+    /* compact, then grow */
+    if (s->qhead > 0) {
+      memmove(s->queue, s->queue + s->qhead,
+              (size_t)(s->qtail - s->qhead) * sizeof(queued_req));
+      s->qtail -= s->qhead;
+      s->qhead = 0;
+    }
+    if (s->qtail == s->qcap) {
+      int64_t nc = s->qcap ? 2 * s->qcap : 256;
+      queued_req *nq = realloc(s->queue, (size_t)nc * sizeof(queued_req));
+      if (!nq)
+        return -1;
+      s->queue = nq;
+      s->qcap = nc;
+    }
+  }
+  if (s->nresults == s->rcap) {
+    int64_t nc = s->rcap ? 2 * s->rcap : 1024;
+    result_rec *nr = realloc(s->results, (size_t)nc * sizeof(result_rec));
+    if (!nr)
+      return -1;
+    s->results = nr;
+    s->rcap = nc;
+  }
+  const int64_t id = s->nresults++;
+  s->results[id].state = 0;
+  s->results[id].price = 0;
+  s->queue[s->qtail].id = id;
+  s->queue[s->qtail].req = *r;
+  s->qtail++;
+  return id;
+}
+
+static int rebuild_tiles(struct lvc_server *s) {
+  struct futhark_context *ctx = s->ctx;
+  int nx_t = 0, ny_t = 0;
+  for (int i = 0; i < s->max_slots; i++) {
+    if (s->slot_id[i] < 0)
+      continue;
+    const lvc_problem *q = &s->h_slots[i];
+    const int rows = LVC_BLOCK / lvc_lanes_for(q->nx, s->seg_x);
+    nx_t += (q->ny + rows - 1) / rows;
+    ny_t += q->ytiles;
+  }
+  const int need = nx_t + ny_t;
+  if (need > s->tiles_cap) {
+    if (s->d_tiles)
+      lvc_dev_free(ctx, s->d_tiles, (size_t)s->tiles_cap * sizeof(lvc_tile));
+    s->d_tiles = NULL;
+    free(s->h_tiles);
+    s->tiles_cap = need + need / 2 + 64;
+    s->h_tiles = malloc((size_t)s->tiles_cap * sizeof(lvc_tile));
+    if (!s->h_tiles ||
+        lvc_dev_alloc(ctx, (size_t)s->tiles_cap * sizeof(lvc_tile), (void **)&s->d_tiles)) {
+      s->tiles_cap = 0;
+      return 1;
+    }
+  }
+  int tx = 0, ty = nx_t;
+  for (int i = 0; i < s->max_slots; i++) {
+    if (s->slot_id[i] < 0)
+      continue;
+    const lvc_problem *q = &s->h_slots[i];
+    const int gx = lvc_lanes_for(q->nx, s->seg_x), gy = lvc_lanes_for(q->ny, s->seg_y);
+    const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
+    for (int r0 = 0; r0 < q->ny; r0 += rows) {
+      lvc_tile t = {i, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx};
+      s->h_tiles[tx++] = t;
+    }
+    for (int c0 = 0; c0 < q->nx; c0 += cols) {
+      lvc_tile t = {i, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy};
+      s->h_tiles[ty++] = t;
+    }
+  }
+  s->nxt = nx_t;
+  s->nyt = ny_t;
+  s->tiles_dirty = 0;
+  return need ? lvc_upload(ctx, s->d_tiles, s->h_tiles, (size_t)need * sizeof(lvc_tile)) : 0;
+}
+
+static int admit(struct lvc_server *s) {
+  struct futhark_context *ctx = s->ctx;
+  int slot = 0;
+  while (s->qhead < s->qtail && s->nresident < s->max_slots) {
+    const queued_req *qr = &s->queue[s->qhead];
+    const int64_t bytes = (int64_t)request_bytes(s, &qr->req);
+    if (s->budget > 0 && s->resident_bytes + bytes > s->budget)
+      break; /* in-order admission: wait for memory */
+    while (s->slot_id[slot] >= 0)
+      slot++;
+    lvc_problem q;
+    memset(&q, 0, sizeof(q));
+    q.nx = qr->req.numX;
+    q.ny = qr->req.numY;
+    q.nt = qr->req.numT;
+    q.s0 = qr->req.s0;
+    q.strike = qr->req.strike;
+    q.t = qr->req.t;
+    q.alpha = qr->req.alpha;
+    q.nu = qr->req.nu;
+    q.beta = qr->req.beta;
+    q.k = -1;
+    q.state_off = (int64_t)slot * s->slot_elems;
+    q.geo_off = (int64_t)slot * s->slot_geo;
+    q.ytiles = (q.nx + LVC_BLOCK / lvc_lanes_for(q.ny, s->seg_y) - 1) /
+               (LVC_BLOCK / lvc_lanes_for(q.ny, s->seg_y));
+    s->h_slots[slot] = q;
+    s->slot_id[slot] = qr->id;
+    s->steps_left[slot] = q.nt - 1;
+    s->results[qr->id].state = 1;
+    s->resident_bytes += bytes;
+    s->nresident++;
+    s->qhead++;
+    s->tiles_dirty = 1;
+    if (lvc_upload(ctx, s->d_slots + slot, &q, sizeof(q)))
+      return 1;
+    int e = lvc_launch_setup((void *)ctx->stream, s->precision, s->d_slots + slot,
+                             1, s->d_geo, s->d_R);
+    if (lvc_check_hip(ctx, e, "setup kernel"))
+      return 1;
+  }
+  return 0;
+}
+
+int32_t lvc_server_step(struct lvc_server *s) {
+  if (!s)
+    return -1;
+  struct futhark_context *ctx = s->ctx;
+  pthread_mutex_lock(&ctx->lock);
+  int rc = admit(s);
+  if (!rc && s->tiles_dirty)
+    rc = rebuild_tiles(s);
+  int32_t ndone = 0;
+  if (!rc && s->nresident > 0) {
+    void *st = (void *)ctx->stream;
+    int tok = lvc_prof_begin(ctx, "lvc_xsweep");
+    int e = lvc_launch_xsweep(st, s->precision, s->seg_x, s->d_slots, s->d_tiles,
+                              s->nxt, s->d_geo, s->d_R, s->d_Y);
+    lvc_prof_end(ctx, tok);
+    rc = lvc_check_hip(ctx, e, "x sweep kernel");
+    if (!rc) {
+      tok = lvc_prof_begin(ctx, "lvc_ysweep");
+      e = lvc_launch_ysweep(st, s->precision, s->seg_y, s->d_slots,
+                            s->d_tiles + s->nxt, s->nyt, s->d_geo, s->d_Y, s->d_R);
+      lvc_prof_end(ctx, tok);
+      rc = lvc_check_hip(ctx, e, "y sweep kernel");
+    }
+    /* completions */
+    for (int i = 0; !rc && i < s->max_slots; i++) {
+      if (s->slot_id[i] < 0)
+        continue;
+      if (--s->steps_left[i] > 0)
+        continue;
+      e = lvc_launch_gather(st, s->precision, s->d_slots + i, 1, s->d_R,
+                            s->d_out + i, 1);
+      rc = lvc_check_hip(ctx, e, "gather kernel");
+      ndone++;
+    }
+    if (!rc && ndone > 0) {
+      rc = lvc_check_hip(ctx, hipMemcpyAsync(s->h_out, s->d_out,
+                                             (size_t)s->max_slots * sizeof(double),
+                                             hipMemcpyDeviceToHost, ctx->stream),
+                         "hipMemcpyAsync") ||
+           lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      for (int i = 0; !rc && i < s->max_slots; i++) {
+        if (s->slot_id[i] < 0 || s->steps_left[i] > 0)
+          continue;
+        const int64_t id = s->slot_id[i];
+        s->results[id].state = 2;
+        s->results[id].price = s->h_out[i];
+        s->resident_bytes -= (int64_t)request_bytes(
+            s, &(struct lvc_request){s->h_slots[i].nx, s->h_slots[i].ny,
+                                     s->h_slots[i].nt, 0, 0, 0, 0, 0, 0});
+        s->slot_id[i] = -1;
+        s->h_slots[i].k = -1;
+        s->nresident--;
+        s->completed++;
+        s->tiles_dirty = 1;
+      }
+    }
+  }
+  pthread_mutex_unlock(&ctx->lock);
+  return rc ? -1 : ndone;
+}
+
+int32_t lvc_server_poll(struct lvc_server *s, int64_t id, double *price) {
+  if (!s || id < 0 || id >= s->nresults)
+    return -1;
+  if (s->results[id].state != 2)
+    return 0;
+  if (price)
+    *price = s->results[id].price;
+  return 1;
+}
+
+int32_t lvc_server_drain(struct lvc_server *s) {
+  if (!s)
+    return -1;
+  while (s->qhead < s->qtail || s->nresident > 0) {
+    const int64_t before_q = s->qhead;
+    const int was_empty = s->nresident == 0;
+    int32_t r = lvc_server_step(s);
+    if (r < 0)
+      return -1;
+    /* with nothing resident at the start of the tick, admission cannot wait
+     * for memory: submit() guaranteed every queued request fits alone */
+    if (was_empty && s->qhead == before_q && s->qhead < s->qtail) {
+      lvc_set_error(s->ctx, "queued request cannot be admitted");
+      return -1;
+    }
+  }
+  return 0;
+}
+
+void lvc_server_stats(struct lvc_server *s, int64_t *queued, int64_t *resident,
+                      int64_t *completed, int64_t *resident_bytes) {
+  if (!s)
+    return;
+  if (queued)
+    *queued = s->qtail - s->qhead;
+  if (resident)
+    *resident = s->nresident;
+  if (completed)
+    *completed = s->completed;
+  if (resident_bytes)
+    *resident_bytes = s->resident_bytes;
+}

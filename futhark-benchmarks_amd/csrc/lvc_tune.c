@@ -1,0 +1,157 @@
+/*
+ * Segment-length autotuner.  The x and y sweeps split each tridiagonal
+ * system into segments of 4, 8 or 16 unknowns per lane (so a 256-point
+ * system uses 64, 32 or 16 lanes and that many partitions in its reduced
+ * system).  Shorter segments mean fewer registers and more lanes (higher
+ * occupancy, deeper reduced-system PCR); longer ones mean less reduced-system
+ * work per unknown.  The choice is made per (precision, numX, numY) class:
+ * from the cache file named by futhark_context_config_set_cache_file when it
+ * has an entry, else by timing the candidates on a small synthetic batch when
+ * "lvc.autotune" is set, else from the built-in table below.
+ *
+ * Determinism: a size class always maps to the same choice within a context,
+ * so a problem's arithmetic never depends on the rest of its batch.
+ *
+ * Cache file format: one line per class, "precision numX numY seg_x seg_y".
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "lvc_runtime.h"
+
+static int table_find(struct futhark_context *ctx, int precision, int nx,
+                      int ny) {
+  for (int i = 0; i < ctx->ntuned; i++)
+    if (ctx->tuned[i].precision == precision && ctx->tuned[i].nx == nx &&
+        ctx->tuned[i].ny == ny)
+      return i;
+  return -1;
+}
+
+static void table_put(struct futhark_context *ctx, int precision, int nx,
+                      int ny, int sx, int sy) {
+  int i = table_find(ctx, precision, nx, ny);
+  if (i < 0) {
+    if (ctx->ntuned == LVC_MAX_TUNED)
+      return;
+    i = ctx->ntuned++;
+  }
+  lvc_tuned t = {precision, nx, ny, sx, sy};
+  ctx->tuned[i] = t;
+}
+
+void lvc_tune_load(struct futhark_context *ctx) {
+  if (!ctx->cfg || !ctx->cfg->cache_file)
+    return;
+  FILE *f = fopen(ctx->cfg->cache_file, "r");
+  if (!f)
+    return;
+  int p, nx, ny, sx, sy;
+  while (fscanf(f, "%d %d %d %d %d", &p, &nx, &ny, &sx, &sy) == 5)
+    if (p >= LVC_FP32 && p <= LVC_FP64 && lvc_seg_supported(sx) &&
+        lvc_seg_supported(sy))
+      table_put(ctx, p, nx, ny, sx, sy);
+  fclose(f);
+}
+
+static void cache_append(struct futhark_context *ctx, int precision, int nx,
+                         int ny, int sx, int sy) {
+  if (!ctx->cfg || !ctx->cfg->cache_file)
+    return;
+  FILE *f = fopen(ctx->cfg->cache_file, "a");
+  if (!f)
+    return;
+  fprintf(f, "%d %d %d %d %d\n", precision, nx, ny, sx, sy);
+  fclose(f);
+}
+
+/* Built-in choice (measured on MI355X, see the tuning table in the docs). */
+static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
+  (void)nx;
+  (void)ny;
+  if (precision == LVC_FP64) {
+    *sx = 4;
+    *sy = 4;
+  } else {
+    *sx = 8;
+    *sy = 8;
+  }
+}
+
+/* Times one candidate: a batch of 8 problems of the class, 4 time steps. */
+static double time_candidate(struct futhark_context *ctx, int precision, int nx,
+                             int ny, int sx, int sy) {
+  lvc_problem probs[8];
+  memset(probs, 0, sizeof(probs));
+  for (int i = 0; i < 8; i++) {
+    probs[i].nx = nx;
+    probs[i].ny = ny;
+    probs[i].nt = 5;
+    probs[i].s0 = 0.03;
+    probs[i].strike = 0.001 * i;
+    probs[i].t = 5.0;
+    probs[i].alpha = 0.2;
+    probs[i].nu = 0.6;
+    probs[i].beta = 0.5;
+  }
+  lvc_batch b;
+  if (lvc_batch_build(ctx, &b, 8, probs, precision, sx, sy))
+    return -1;
+  hipEvent_t e0 = NULL, e1 = NULL;
+  double best = -1;
+  if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+    for (int rep = 0; rep < 4; rep++) {
+      (void)hipEventRecord(e0, ctx->stream);
+      if (lvc_batch_run(ctx, &b))
+        break;
+      (void)hipEventRecord(e1, ctx->stream);
+      if (hipEventSynchronize(e1) != hipSuccess)
+        break;
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0 && (best < 0 || ms < best))
+        best = ms;
+    }
+  }
+  if (e0)
+    (void)hipEventDestroy(e0);
+  if (e1)
+    (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(ctx->stream);
+  lvc_batch_release(ctx, &b);
+  return best;
+}
+
+void lvc_tune_pick(struct futhark_context *ctx, int precision, int nx, int ny,
+                   int *seg_x, int *seg_y) {
+  int i = table_find(ctx, precision, nx, ny);
+  if (i >= 0) {
+    *seg_x = ctx->tuned[i].seg_x;
+    *seg_y = ctx->tuned[i].seg_y;
+    return;
+  }
+  builtin(precision, nx, ny, seg_x, seg_y);
+  if (!ctx->cfg || !ctx->cfg->autotune || !ctx->ok)
+    return;
+  static const int segs[] = {4, 8, 16};
+  double best = -1;
+  int bx = *seg_x, by = *seg_y;
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) {
+      if (lvc_lanes_for(nx, segs[a]) > 64 || lvc_lanes_for(ny, segs[c]) > 64)
+        continue;
+      double ms = time_candidate(ctx, precision, nx, ny, segs[a], segs[c]);
+      if (ms > 0 && (best < 0 || ms < best)) {
+        best = ms;
+        bx = segs[a];
+        by = segs[c];
+      }
+    }
+  char *err = futhark_context_get_error(ctx); /* a failed trial is not fatal */
+  free(err);
+  *seg_x = bx;
+  *seg_y = by;
+  table_put(ctx, precision, nx, ny, bx, by);
+  cache_append(ctx, precision, nx, ny, bx, by);
+}

@@ -1,0 +1,228 @@
+/*
+ * locvolcalib.h -- C ABI of the MI355X LocVolCalib library.
+ *
+ * Drop-in for the C API that `futhark <backend> --library` generates for
+ * finpar/LocVolCalib.fut.  The generated header is not part of the reference
+ * repository; the names, argument order and ownership rules below follow the
+ * generator's convention as exercised by the reference's own C callers (the
+ * vendored lys frontends, e.g. accelerate/julia/lib/github.com/diku-dk/lys/
+ * shared.c:15-49, console/liblys.c:243-251, console/main.c:176-180,
+ * shared.h:20-28).  Each declaration cites the caller line it serves or the
+ * LocVolCalib.fut line that fixes its signature.
+ *
+ * Conventions (as in the generated API):
+ *  - every fallible call returns int: 0 = success, non-zero = failure; the
+ *    message is then available from futhark_context_get_error();
+ *  - arrays are opaque handles owned by the caller, holding device memory;
+ *    futhark_values_* copies them to host memory;
+ *  - entry points are asynchronous with respect to the host; results are
+ *    valid after futhark_context_sync() (futhark_values_* synchronises);
+ *  - the configuration must outlive the context and is freed after it;
+ *  - one context per device; calls on one context are serialised by an
+ *    internal lock.
+ *
+ * Everything here is plain C: pointers, sizes and scalars, no HIP or torch
+ * types.  Device pointers are exposed as `void *`.
+ */
+#ifndef LOCVOLCALIB_H
+#define LOCVOLCALIB_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct futhark_context_config;
+struct futhark_context;
+struct futhark_f32_1d;
+struct futhark_f64_1d;
+struct futhark_i32_1d;
+
+/* ---- configuration ------------------------------------------------------ */
+
+/* lys shared.c:20 */
+struct futhark_context_config *futhark_context_config_new(void);
+/* lys console/main.c:180 */
+void futhark_context_config_free(struct futhark_context_config *cfg);
+/* lys shared.c:25.  Accepts "hip:N", "#N" or "N" (device ordinal). */
+void futhark_context_config_set_device(struct futhark_context_config *cfg,
+                                       const char *s);
+/* lys shared.c:44.  Here: the autotuner's cache file (read at context
+ * creation when it exists, written after tuning). */
+void futhark_context_config_set_cache_file(struct futhark_context_config *cfg,
+                                           const char *f);
+/* Generated API (set_tuning_param; the reference's knobs are in
+ * finpar/LocVolCalib.fut.tuning:1-2).  Recognised names:
+ *   "main.suff_intra_par_0", "main.suff_intra_par_1"  accepted, no effect on
+ *       results (the reference's version-selection thresholds);
+ *   "lvc.precision"  0 = FP32, 1 = FP32 + one FP64-residual refinement per
+ *       implicit solve (default), 2 = FP64 (for futhark_entry_main);
+ *   "lvc.seg_x", "lvc.seg_y"  segment length of the x / y sweeps (2..32,
+ *       power of two; 0 = autotuned);
+ *   "lvc.autotune"  1 = time the candidate segment lengths on first use of
+ *       a size class (default 0: built-in table or cache file).
+ * Returns non-zero for an unknown name. */
+int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
+                                            const char *param_name,
+                                            size_t new_value);
+void futhark_context_config_set_debugging(struct futhark_context_config *cfg,
+                                          int flag);
+/* Per-kernel timing with HIP events; read back with futhark_context_report. */
+void futhark_context_config_set_profiling(struct futhark_context_config *cfg,
+                                          int flag);
+void futhark_context_config_set_logging(struct futhark_context_config *cfg,
+                                        int flag);
+
+int futhark_get_tuning_param_count(void);
+const char *futhark_get_tuning_param_name(int i);
+const char *futhark_get_tuning_param_class(int i);
+
+/* ---- context ------------------------------------------------------------ */
+
+/* lys shared.c:47.  Never returns NULL except on host allocation failure; a
+ * device that cannot be opened is reported by futhark_context_get_error()
+ * and makes every later call fail. */
+struct futhark_context *futhark_context_new(struct futhark_context_config *cfg);
+/* lys console/main.c:179 */
+void futhark_context_free(struct futhark_context *ctx);
+/* lys console/liblys.c:249 */
+int futhark_context_sync(struct futhark_context *ctx);
+/* lys shared.h:25.  Returns a malloc'd message (caller frees) or NULL, and
+ * clears the stored error. */
+char *futhark_context_get_error(struct futhark_context *ctx);
+void futhark_context_set_logging_file(struct futhark_context *ctx, FILE *f);
+/* JSON object: per-kernel {"count", "total_us"} collected while profiling is
+ * enabled.  Caller frees. */
+char *futhark_context_report(struct futhark_context *ctx);
+void futhark_context_pause_profiling(struct futhark_context *ctx);
+void futhark_context_unpause_profiling(struct futhark_context *ctx);
+/* Releases cached device memory held by the context's allocator. */
+int futhark_context_clear_caches(struct futhark_context *ctx);
+
+/* ---- arrays (cf. lys console/liblys.c:248-250, shared.c:79) ------------- */
+
+struct futhark_f32_1d *futhark_new_f32_1d(struct futhark_context *ctx,
+                                          const float *data, int64_t dim0);
+/* Wraps existing device memory (not copied, not freed by the library). */
+struct futhark_f32_1d *futhark_new_raw_f32_1d(struct futhark_context *ctx,
+                                              void *data, int64_t dim0);
+int futhark_free_f32_1d(struct futhark_context *ctx, struct futhark_f32_1d *arr);
+int futhark_values_f32_1d(struct futhark_context *ctx, struct futhark_f32_1d *arr,
+                          float *data);
+int futhark_index_f32_1d(struct futhark_context *ctx, float *out,
+                         struct futhark_f32_1d *arr, int64_t i0);
+void *futhark_values_raw_f32_1d(struct futhark_context *ctx,
+                                struct futhark_f32_1d *arr);
+const int64_t *futhark_shape_f32_1d(struct futhark_context *ctx,
+                                    struct futhark_f32_1d *arr);
+
+struct futhark_f64_1d *futhark_new_f64_1d(struct futhark_context *ctx,
+                                          const double *data, int64_t dim0);
+struct futhark_f64_1d *futhark_new_raw_f64_1d(struct futhark_context *ctx,
+                                              void *data, int64_t dim0);
+int futhark_free_f64_1d(struct futhark_context *ctx, struct futhark_f64_1d *arr);
+int futhark_values_f64_1d(struct futhark_context *ctx, struct futhark_f64_1d *arr,
+                          double *data);
+int futhark_index_f64_1d(struct futhark_context *ctx, double *out,
+                         struct futhark_f64_1d *arr, int64_t i0);
+void *futhark_values_raw_f64_1d(struct futhark_context *ctx,
+                                struct futhark_f64_1d *arr);
+const int64_t *futhark_shape_f64_1d(struct futhark_context *ctx,
+                                    struct futhark_f64_1d *arr);
+
+struct futhark_i32_1d *futhark_new_i32_1d(struct futhark_context *ctx,
+                                          const int32_t *data, int64_t dim0);
+int futhark_free_i32_1d(struct futhark_context *ctx, struct futhark_i32_1d *arr);
+int futhark_values_i32_1d(struct futhark_context *ctx, struct futhark_i32_1d *arr,
+                          int32_t *data);
+const int64_t *futhark_shape_i32_1d(struct futhark_context *ctx,
+                                    struct futhark_i32_1d *arr);
+
+/* ---- entry points ------------------------------------------------------- */
+
+/* LocVolCalib.fut:238-252 (`main`): prices of `outer` options with strikes
+ * 0.001*i, i = 0..outer-1, all sharing the grid sizes and parameters.
+ * Computed in the precision set by "lvc.precision" (default FP32 with one
+ * FP64-residual refinement per implicit solve).  Errors: numX or numY < 3 or
+ * numT < 2 (the reference's range `1...n-2` at :37 and its timeline), outer
+ * < 0 (`iota` at :247), device failure.  outer == 0 returns an empty array. */
+int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0,
+                       const int32_t outer_loop_count, const int32_t numX,
+                       const int32_t numY, const int32_t numT, const float s0,
+                       const float t, const float alpha, const float nu,
+                       const float beta);
+
+/* The same program with every real f64 (the FP64 plain form): the f32 ->
+ * f64 type substitution of LocVolCalib.fut:238-252.  No counterpart in the
+ * reference, which is f32-only; named as the generator names an extra
+ * `entry main_f64`. */
+int futhark_entry_main_f64(struct futhark_context *ctx,
+                           struct futhark_f64_1d **out0,
+                           const int32_t outer_loop_count, const int32_t numX,
+                           const int32_t numY, const int32_t numT,
+                           const double s0, const double t, const double alpha,
+                           const double nu, const double beta);
+
+/* Batched form: problem p has its own sizes (numX[p], numY[p], numT[p]) and
+ * parameters and runs LocVolCalib.fut:211-236 (`value`).  All arrays have
+ * the same length n.  `precision` as "lvc.precision".  The problems advance
+ * together, one launch of each sweep per time step (replayed from a HIP
+ * graph), until each has done its own numT-1 steps.  No counterpart in the
+ * reference (its `main` shares one size and parameter set). */
+int futhark_entry_main_batch(struct futhark_context *ctx,
+                             struct futhark_f64_1d **out0,
+                             const struct futhark_i32_1d *numX,
+                             const struct futhark_i32_1d *numY,
+                             const struct futhark_i32_1d *numT,
+                             const struct futhark_f64_1d *s0,
+                             const struct futhark_f64_1d *strike,
+                             const struct futhark_f64_1d *t,
+                             const struct futhark_f64_1d *alpha,
+                             const struct futhark_f64_1d *nu,
+                             const struct futhark_f64_1d *beta,
+                             int32_t precision);
+
+/* ---- serving (continuous batching) -------------------------------------
+ * No counterpart in the reference.  A server owns a set of resident slots on
+ * the context's device: each admitted request keeps its grid in device memory
+ * and advances one time step per engine tick together with every other
+ * resident request; finished requests free their slot and memory, and queued
+ * requests are admitted as slots and the memory budget allow.  A request's
+ * result depends only on its own inputs, never on what else is resident. */
+struct lvc_server;
+
+struct lvc_request {
+  int32_t numX, numY, numT;
+  double s0, strike, t, alpha, nu, beta;
+};
+
+/* max_slots: resident requests at most; mem_budget_bytes: device memory the
+ * resident grids may use (0 = no limit besides max_slots); precision as
+ * "lvc.precision". */
+struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots,
+                                  int64_t mem_budget_bytes, int32_t precision);
+void lvc_server_free(struct lvc_server *srv);
+/* Queues a request; returns its id (>= 0) or -1 (invalid request or a
+ * request that can never fit the memory budget; see get_error). */
+int64_t lvc_server_submit(struct lvc_server *srv, const struct lvc_request *req);
+/* Runs one engine tick: admits queued requests, advances every resident
+ * request by one time step, completes the ones that finished.  Returns the
+ * number of requests completed by this tick, or -1 on error. */
+int32_t lvc_server_step(struct lvc_server *srv);
+/* Completed result for `id`: returns 1 and stores the price, 0 when not yet
+ * complete, -1 for an unknown id. */
+int32_t lvc_server_poll(struct lvc_server *srv, int64_t id, double *price);
+/* Ticks until nothing is queued or resident.  Returns 0 or -1. */
+int32_t lvc_server_drain(struct lvc_server *srv);
+/* Counters: queued, resident, completed, resident bytes. */
+void lvc_server_stats(struct lvc_server *srv, int64_t *queued, int64_t *resident,
+                      int64_t *completed, int64_t *resident_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOCVOLCALIB_H */

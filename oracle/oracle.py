@@ -1,0 +1,233 @@
+"""CPU oracle for the LocVolCalib hot path -- TEST INFRASTRUCTURE ONLY.
+
+Python (ctypes) access to ``oracle/_build/liblvc_oracle.so``, a plain-C
+restatement of ``finpar/LocVolCalib.fut`` (see ``lvc_oracle_impl.h``).  Only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker; the product
+library never routes through it.
+
+Functions and what pins each one (the checks live in ``tests/test_oracle.py``):
+
+=====================  =====================================  ==========================================
+oracle function        reference                              pinned by
+=====================  =====================================  ==========================================
+init_grid              LocVolCalib.fut:12-23                  closed forms: x[xindex] == s0, y[NY/2] ==
+                                                              log(alpha), uniform spacing, timeline ends
+init_operator          LocVolCalib.fut:26-44                  exact identities: D*1 = 0, D*x = 1,
+                                                              DD*x = 0, DD*x^2 = 2, boundary literals
+set_payoff             LocVolCalib.fut:46-47                  direct formula
+update_params          LocVolCalib.fut:50-66                  separable closed form
+                                                              x^(2 beta) exp(2y - nu^2 t)
+tridag_par             LocVolCalib.fut:68-124                 dense solve (numpy.linalg.solve) for
+                                                              n <= 16; agreement with tridag_seq;
+                                                              ocean-sim golden vectors (tridiag32-small)
+tridag_seq             misc/ocean-sim/tridiag.fut:81-95       dense solve; ocean-sim golden vectors
+explicit_method        LocVolCalib.fut:126-153                dense matrix-vector product on tiny grids
+value / main (f32)     LocVolCalib.fut:211-252                the reference's fixtures small/medium/large
+                                                              (.in -> .out), closed form price = s0 - K
+                                                              for K < x_min, monotone/convex in K
+value / main (f64)     f32 -> f64 type substitution of the    closed form, monotone/convex in K, f64 vs
+                       same lines                             the f32 fixtures at fixture precision
+=====================  =====================================  ==========================================
+
+Below ~1e-6 absolute the f32 bit pattern of the Futhark C backend (its fusion
+order and libm calls) is not pinned by anything in the reference; the FP64
+restatement is pinned only by the closed form and invariants above.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liblvc_oracle.so")
+_lib = None
+
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_ptr = ctypes.c_void_p
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc) and return the .so path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        for suf, real in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
+            f = getattr(_lib, "lvc_value" + suf)
+            f.restype = real
+            f.argtypes = [_c_i64, _c_i64, _c_i64] + [real] * 6 + [ctypes.c_int, _ptr]
+            f = getattr(_lib, "lvc_main" + suf)
+            f.restype = ctypes.c_int
+            f.argtypes = [_c_i32] * 4 + [real] * 5 + [ctypes.c_int, ctypes.c_int, _ptr]
+            f = getattr(_lib, "lvc_batch" + suf)
+            f.restype = ctypes.c_int
+            f.argtypes = [_c_i32] + [_ptr] * 9 + [ctypes.c_int, ctypes.c_int, _ptr]
+            f = getattr(_lib, "lvc_init_grid" + suf)
+            f.restype = None
+            f.argtypes = [real] * 4 + [_c_i64] * 3 + [_ptr] * 5
+            f = getattr(_lib, "lvc_init_operator" + suf)
+            f.restype = None
+            f.argtypes = [_ptr, _c_i64, _ptr, _ptr]
+            f = getattr(_lib, "lvc_set_payoff" + suf)
+            f.restype = None
+            f.argtypes = [real, _ptr, _c_i64, _c_i64, _ptr]
+            f = getattr(_lib, "lvc_update_params" + suf)
+            f.restype = None
+            f.argtypes = [_ptr, _ptr, _c_i64, _c_i64, real, real, real, _ptr]
+            for name in ("lvc_tridag_par", "lvc_tridag_seq"):
+                f = getattr(_lib, name + suf)
+                f.restype = None
+                f.argtypes = [_ptr] * 4 + [_c_i64, _ptr, _ptr]
+            f = getattr(_lib, "lvc_explicit" + suf)
+            f.restype = None
+            f.argtypes = [_ptr] * 5 + [_c_i64, _c_i64, _ptr]
+        _lib.lvc_oracle_set_mutation.argtypes = [ctypes.c_int]
+        _lib.lvc_oracle_set_mutation.restype = None
+    return _lib
+
+
+def _suffix(dtype) -> tuple[str, type]:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "_f32", np.float32
+    if dt == np.float64:
+        return "_f64", np.float64
+    raise ValueError(f"oracle supports float32/float64, not {dt}")
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def set_mutation(m: int) -> None:
+    """Plant mistake number ``m`` in the oracle (0 = none).  Tests only."""
+    lib().lvc_oracle_set_mutation(int(m))
+
+
+def main(outer, numX, numY, numT, s0, t, alpha, nu, beta, dtype=np.float32,
+         thomas=False, nthreads=1) -> np.ndarray:
+    """LocVolCalib.fut:238-252: prices of ``outer`` options, strike 0.001*i."""
+    suf, real = _suffix(dtype)
+    out = np.zeros(max(int(outer), 0), dtype=real)
+    rc = getattr(lib(), "lvc_main" + suf)(outer, numX, numY, numT, s0, t, alpha,
+                                          nu, beta, int(thomas), int(nthreads),
+                                          _p(out) if out.size else None)
+    if rc != 0:
+        raise ValueError(f"oracle main failed with code {rc}")
+    return out
+
+
+def value(numX, numY, numT, s0, strike, t, alpha, nu, beta, dtype=np.float64,
+          thomas=False) -> float:
+    """LocVolCalib.fut:211-236 for one option."""
+    suf, real = _suffix(dtype)
+    err = ctypes.c_int(0)
+    v = getattr(lib(), "lvc_value" + suf)(numX, numY, numT, s0, strike, t,
+                                           alpha, nu, beta, int(thomas),
+                                           ctypes.addressof(err))
+    if err.value:
+        raise ValueError(f"oracle value failed with code {err.value}")
+    return float(v)
+
+
+def batch(numX, numY, numT, s0, strike, t, alpha, nu, beta, dtype=np.float64,
+          thomas=False, nthreads=1) -> np.ndarray:
+    """Per-problem sizes and parameters; same arithmetic as ``value``."""
+    suf, real = _suffix(dtype)
+    n = len(numX)
+    ints = [np.ascontiguousarray(v, dtype=np.int32) for v in (numX, numY, numT)]
+    reals = [np.ascontiguousarray(v, dtype=real) for v in (s0, strike, t, alpha, nu, beta)]
+    out = np.zeros(n, dtype=real)
+    if n == 0:
+        return out
+    rc = getattr(lib(), "lvc_batch" + suf)(n, *[_p(a) for a in ints],
+                                           *[_p(a) for a in reals], int(thomas),
+                                           int(nthreads), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle batch failed with code {rc}")
+    return out
+
+
+def init_grid(s0, alpha, nu, t, numX, numY, numT, dtype=np.float64):
+    """LocVolCalib.fut:12-23 -> (xindex, yindex, x, y, timeline)."""
+    suf, real = _suffix(dtype)
+    x = np.zeros(numX, real)
+    y = np.zeros(numY, real)
+    tl = np.zeros(numT, real)
+    xi = np.zeros(1, np.int32)
+    yi = np.zeros(1, np.int32)
+    getattr(lib(), "lvc_init_grid" + suf)(s0, alpha, nu, t, numX, numY, numT,
+                                          _p(xi), _p(yi), _p(x), _p(y), _p(tl))
+    return int(xi[0]), int(yi[0]), x, y, tl
+
+
+def init_operator(x: np.ndarray):
+    """LocVolCalib.fut:26-44 -> (D [n][3], DD [n][3])."""
+    suf, real = _suffix(x.dtype)
+    x = np.ascontiguousarray(x)
+    n = x.shape[0]
+    D = np.zeros((n, 3), real)
+    DD = np.zeros((n, 3), real)
+    getattr(lib(), "lvc_init_operator" + suf)(_p(x), n, _p(D), _p(DD))
+    return D, DD
+
+
+def set_payoff(strike, x: np.ndarray, numY: int) -> np.ndarray:
+    suf, real = _suffix(x.dtype)
+    x = np.ascontiguousarray(x)
+    out = np.zeros((numY, x.shape[0]), real)
+    getattr(lib(), "lvc_set_payoff" + suf)(strike, _p(x), x.shape[0], numY, _p(out))
+    return out
+
+
+def update_params(x: np.ndarray, y: np.ndarray, tnow, beta, nu) -> np.ndarray:
+    """VarX [numY][numX] of LocVolCalib.fut:61-65."""
+    suf, real = _suffix(x.dtype)
+    x = np.ascontiguousarray(x)
+    y = np.ascontiguousarray(y, dtype=real)
+    out = np.zeros((y.shape[0], x.shape[0]), real)
+    getattr(lib(), "lvc_update_params" + suf)(_p(x), _p(y), x.shape[0], y.shape[0],
+                                              tnow, beta, nu, _p(out))
+    return out
+
+
+def _tridag(name, a, b, c, y):
+    suf, real = _suffix(np.asarray(y).dtype)
+    a, b, c, y = (np.ascontiguousarray(v, dtype=real) for v in (a, b, c, y))
+    n = y.shape[0]
+    out = np.zeros(n, real)
+    tmp = np.zeros(5 * n, real)
+    getattr(lib(), name + suf)(_p(a), _p(b), _p(c), _p(y), n, _p(out), _p(tmp))
+    return out
+
+
+def tridag_par(a, b, c, y) -> np.ndarray:
+    """LocVolCalib.fut:68-124, scan form."""
+    return _tridag("lvc_tridag_par", a, b, c, y)
+
+
+def tridag_seq(a, b, c, y) -> np.ndarray:
+    """misc/ocean-sim/tridiag.fut:81-95, Thomas."""
+    return _tridag("lvc_tridag_seq", a, b, c, y)
+
+
+def explicit_method(D, DD, mu, var, res) -> np.ndarray:
+    """LocVolCalib.fut:126-153 on a [rows][m] operand."""
+    suf, real = _suffix(np.asarray(res).dtype)
+    D, DD, mu, var, res = (np.ascontiguousarray(v, dtype=real) for v in (D, DD, mu, var, res))
+    rows, m = res.shape
+    out = np.zeros((rows, m), real)
+    getattr(lib(), "lvc_explicit" + suf)(_p(D), _p(DD), _p(mu), _p(var), _p(res),
+                                         rows, m, _p(out))
+    return out
