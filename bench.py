@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain device copy of 256 MiB (read) + 256 MiB "
+                         "(write) three times, to scale HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -95,6 +98,16 @@ def main():
 
     import numpy as np
     import locvolcalib as lvc
+
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     with lvc.LocVolCalib(device=0, profiling=True) as ctx:
         # FP32 + refinement (headline)

@@ -63,6 +63,20 @@ void lvc_choose_segs(struct futhark_context *ctx, int precision, int nx, int ny,
 
 static int64_t align64(int64_t v) { return (v + 63) & ~(int64_t)63; }
 
+/* Stable insertion sort of problem indices by numT, descending (runs in
+ * O(n) for the common case of equal or already ordered numT). */
+static void nt_sort_desc(int *order, int n, const lvc_problem *probs) {
+  for (int i = 1; i < n; i++) {
+    const int v = order[i];
+    int j = i - 1;
+    while (j >= 0 && probs[order[j]].nt < probs[v].nt) {
+      order[j + 1] = order[j];
+      j--;
+    }
+    order[j + 1] = v;
+  }
+}
+
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
                     const lvc_problem *probs, int precision, int seg_x,
                     int seg_y) {
@@ -90,6 +104,8 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     q.status = 0;
     q.ydone = 0;
     q.ytiles = (q.nx + cols - 1) / cols;
+    q.xtiles = (q.ny + rows - 1) / rows;
+    q.xdone = 0;
     geo += align64(LVC_GEO_LEN(q.nx, q.ny));
     state += align64((int64_t)q.nx * q.ny);
     nxt += (q.ny + rows - 1) / rows;
@@ -106,12 +122,25 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   b->nyt = (int)nyt;
   b->steps = steps;
   lvc_tile *tiles = malloc((size_t)(nxt + nyt) * sizeof(lvc_tile));
-  if (!tiles) {
+  int *order = malloc((size_t)n * sizeof(int));
+  b->x_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
+  b->y_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
+  if (!tiles || !order || !b->x_active || !b->y_active) {
+    free(tiles);
+    free(order);
     lvc_set_error(ctx, "host allocation failed");
     return 1;
   }
+  /* Problems by numT, longest first (stable), so that at every step the
+   * tiles of the problems still running form a prefix of each list. */
+  for (int p = 0; p < n; p++)
+    order[p] = p;
+  nt_sort_desc(order, n, b->h_probs);
   int64_t tx = 0, ty = nxt;
-  for (int p = 0; p < n; p++) {
+  for (int64_t s = 0; s < steps; s++)
+    b->x_active[s] = b->y_active[s] = 0;
+  for (int oi = 0; oi < n; oi++) {
+    const int p = order[oi];
     const lvc_problem *q = &b->h_probs[p];
     const int gx = lvc_lanes_for(q->nx, seg_x), gy = lvc_lanes_for(q->ny, seg_y);
     const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
@@ -123,7 +152,13 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
       lvc_tile t = {p, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy};
       tiles[ty++] = t;
     }
+    /* this problem runs steps 0 .. nt-2: the prefixes reach past its tiles */
+    for (int64_t s = 0; s < q->nt - 1; s++) {
+      b->x_active[s] = (int)tx;
+      b->y_active[s] = (int)(ty - nxt);
+    }
   }
+  free(order);
   b->probs_bytes = (size_t)n * sizeof(lvc_problem);
   b->geo_bytes = (size_t)geo * sizeof(double);
   b->state_bytes = (size_t)state * real;
@@ -137,6 +172,34 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     rc = lvc_upload(ctx, b->d_probs, b->h_probs, b->probs_bytes) ||
          lvc_upload(ctx, b->d_tiles, tiles, b->tiles_bytes);
   free(tiles);
+  /* fused step: one persistent launch per step needs equal segment lengths */
+  b->fused = ctx->cfg && ctx->cfg->fused && seg_x == seg_y && steps > 0;
+  b->graph = ctx->cfg && ctx->cfg->graph;
+  if (!rc && b->fused) {
+    int32_t *act = malloc((size_t)steps * 2 * sizeof(int32_t));
+    if (!act) {
+      lvc_set_error(ctx, "host allocation failed");
+      return 1;
+    }
+    int most = 0;
+    for (int64_t st = 0; st < steps; st++) {
+      act[2 * st] = b->x_active[st];
+      act[2 * st + 1] = b->y_active[st];
+      if (act[2 * st] + act[2 * st + 1] > most)
+        most = act[2 * st] + act[2 * st + 1];
+    }
+    /* persistent grid: what the device holds at once, at most one workgroup
+     * per tile */
+    const int cap = 4 * (ctx->num_cus > 0 ? ctx->num_cus : 256);
+    b->grid = most < cap ? most : cap;
+    const lvc_ctl zero = {0, 0, 0, 0};
+    rc = lvc_dev_alloc(ctx, (size_t)steps * 2 * sizeof(int32_t), (void **)&b->d_active) ||
+         lvc_dev_alloc(ctx, sizeof(lvc_ctl), (void **)&b->d_ctl);
+    if (!rc)
+      rc = lvc_upload(ctx, b->d_active, act, (size_t)steps * 2 * sizeof(int32_t)) ||
+           lvc_upload(ctx, b->d_ctl, &zero, sizeof(zero));
+    free(act);
+  }
   return rc;
 }
 
@@ -146,8 +209,70 @@ void lvc_batch_release(struct futhark_context *ctx, lvc_batch *b) {
   lvc_dev_free(ctx, b->d_R, b->state_bytes);
   lvc_dev_free(ctx, b->d_Y, b->state_bytes);
   lvc_dev_free(ctx, b->d_tiles, b->tiles_bytes);
+  if (b->d_active)
+    lvc_dev_free(ctx, b->d_active, (size_t)b->steps * 2 * sizeof(int32_t));
+  if (b->d_ctl)
+    lvc_dev_free(ctx, b->d_ctl, sizeof(lvc_ctl));
   free(b->h_probs);
+  free(b->x_active);
+  free(b->y_active);
   memset(b, 0, sizeof(*b));
+}
+
+/* Fused path: `steps` launches of the persistent step kernel, replayed from
+ * a one-node HIP graph when enabled (the launch reads its step index from
+ * the control block, so every launch is the same).  A bounded wait that ran
+ * out inside the kernel is reported here. */
+static int run_fused(struct futhark_context *ctx, lvc_batch *b,
+                     const lvc_tile *xt, const lvc_tile *yt) {
+  void *st = (void *)ctx->stream;
+  hipGraph_t graph = NULL;
+  hipGraphExec_t exec = NULL;
+  int rc = 0;
+  if (b->graph) {
+    rc = lvc_check_hip(ctx, hipStreamBeginCapture(ctx->stream,
+                                                  hipStreamCaptureModeThreadLocal),
+                       "hipStreamBeginCapture");
+    if (!rc) {
+      int e = lvc_launch_step(st, b->precision, b->seg_x, b->seg_y, b->d_probs, xt,
+                              yt, b->d_active, b->d_ctl, b->grid, b->d_geo, b->d_R,
+                              b->d_Y);
+      int e2 = (int)hipStreamEndCapture(ctx->stream, &graph);
+      rc = lvc_check_hip(ctx, e, "step kernel (capture)") ||
+           lvc_check_hip(ctx, e2, "hipStreamEndCapture") ||
+           lvc_check_hip(ctx, hipGraphInstantiate(&exec, graph, NULL, NULL, 0),
+                         "hipGraphInstantiate");
+    }
+  }
+  for (int64_t s = 0; !rc && s < b->steps; s++) {
+    int tok = lvc_prof_begin(ctx, "lvc_step");
+    int e = exec ? (int)hipGraphLaunch(exec, ctx->stream)
+                 : lvc_launch_step(st, b->precision, b->seg_x, b->seg_y, b->d_probs,
+                                   xt, yt, b->d_active, b->d_ctl, b->grid, b->d_geo,
+                                   b->d_R, b->d_Y);
+    lvc_prof_end(ctx, tok);
+    rc = lvc_check_hip(ctx, e, "step kernel");
+  }
+  lvc_ctl ctl = {0, 0, 0, 0};
+  if (!rc)
+    rc = lvc_check_hip(ctx, hipMemcpyAsync(&ctl, b->d_ctl, sizeof(ctl),
+                                           hipMemcpyDeviceToHost, ctx->stream),
+                       "hipMemcpyAsync") ||
+         lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  if (exec)
+    (void)hipGraphExecDestroy(exec);
+  if (graph)
+    (void)hipGraphDestroy(graph);
+  if (!rc && ctl.error) {
+    lvc_set_error(ctx, "fused step: a wait for x tiles expired");
+    rc = 1;
+  }
+  if (!rc && ctl.step != b->steps) {
+    lvc_set_error(ctx, "fused step: %d of %lld steps ran", ctl.step,
+                  (long long)b->steps);
+    rc = 1;
+  }
+  return rc;
 }
 
 int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
@@ -160,16 +285,18 @@ int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
   if (lvc_check_hip(ctx, e, "setup kernel"))
     return 1;
   const lvc_tile *xt = b->d_tiles, *yt = b->d_tiles + b->nxt;
+  if (b->fused)
+    return run_fused(ctx, b, xt, yt);
   for (int64_t s = 0; s < b->steps; s++) {
     tok = lvc_prof_begin(ctx, "lvc_xsweep");
-    e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt, b->nxt,
-                          b->d_geo, b->d_R, b->d_Y);
+    e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
+                          b->x_active[s], b->d_geo, b->d_R, b->d_Y);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "x sweep kernel"))
       return 1;
     tok = lvc_prof_begin(ctx, "lvc_ysweep");
-    e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt, b->nyt,
-                          b->d_geo, b->d_Y, b->d_R);
+    e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
+                          b->y_active[s], b->d_geo, b->d_Y, b->d_R);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "y sweep kernel"))
       return 1;

@@ -24,18 +24,20 @@
 // (a, c <= 0), solved without pivoting.
 //
 // Tridiagonal solve (one system = one row or column of length n): the system
-// is cut into segments of SEG consecutive unknowns, one segment per lane, and
-// the `lanes` (a power of two >= ceil(n/SEG)) lanes of a system are
+// is cut into segments of M consecutive unknowns, one segment per lane, and
+// the `lanes` (a power of two >= ceil(n/M), at most 64) lanes of a system are
 // consecutive lanes of one wavefront.  Each lane eliminates its segment
-// locally (Thomas on the first SEG-1 rows, written so that every pivot is a
-// sum of positive terms), which leaves one unknown per lane (its last) in a
+// locally (Thomas on the first M-1 rows, written so that every pivot is a sum
+// of positive terms), which leaves one unknown per lane (its last) in a
 // tridiagonal reduced system; that is solved by parallel cyclic reduction
-// with wavefront shuffles, and each lane then back-substitutes.  Lanes past
-// the end of the system hold identity rows.
+// across the lanes, and each lane then back-substitutes.  When a system's
+// lanes fit one 16-lane row the PCR exchanges are DPP row shifts (VALU
+// operand modifiers, no LDS round trip); wider systems use ds_bpermute.
+// Lanes past the end of the system hold identity rows.
 //
 // Precision: T = float or double for the state and the sweeps.  REF adds one
 // iterative-refinement pass to each implicit solve: the residual of the FP32
-// solution is formed in FP64 (with the FP64 taps and the same rounded time
+// solution is formed in FP64 (with the same taps and the same rounded time
 // step dtInv that the explicit part and the combination use, so that the
 // scheme stays consistent), solved again in FP32 with the same
 // factorisation, and added in FP64.
@@ -49,66 +51,8 @@
 namespace {
 
 // ---------------------------------------------------------------------------
-// Setup: initGrid (:12-23), initOperator (:26-44) taps, the VarX factor
-// x^(2 beta) of updateParams (:61-65) and setPayoff (:46-47).  Computed in
-// FP64 with multiply-add contraction disabled, so the grid is bit-identical
-// to a C restatement of the same FP64 lines.
+// Small helpers
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
-                                                        double *geo, T *R) {
-#pragma clang fp contract(off)
-  lvc_problem &pr = probs[blockIdx.x];
-  const int nx = pr.nx, ny = pr.ny, nt = pr.nt;
-  const double s0 = pr.s0, t = pr.t, alpha = pr.alpha, nu = pr.nu, beta = pr.beta;
-  double *x = geo + pr.geo_off;
-  double *ax = x + nx, *cx = ax + nx, *y = cx + nx, *ay = y + ny, *cy = ay + ny;
-
-  const double logAlpha = log(alpha);
-  const double stdX = 20.0 * alpha * s0 * sqrt(t);
-  const double stdY = 10.0 * nu * sqrt(t);
-  const double dx = stdX / (double)nx, dy = stdY / (double)ny;
-  const int xi = (int)(s0 / dx);  // i32.f64: truncation toward zero
-  const int yi = ny / 2;
-  for (int i = threadIdx.x; i < nx; i += blockDim.x)
-    x[i] = (double)i * dx - (double)xi * dx + s0;
-  for (int j = threadIdx.x; j < ny; j += blockDim.x)
-    y[j] = (double)j * dy - (double)yi * dy + logAlpha;
-  if (threadIdx.x == 0) {
-    pr.xindex = xi;
-    pr.yindex = yi;
-    pr.k = nt - 2;
-    pr.status = (xi >= 0 && xi < nx) ? 0 : 3;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-    double d0 = 0.0, d2 = 0.0;
-    if (i >= 1 && i <= nx - 2) {
-      const double dl = x[i] - x[i - 1], du = x[i + 1] - x[i];
-      d0 = 2.0 / dl / (dl + du);
-      d2 = 2.0 / du / (dl + du);
-    }
-    const double X = exp(2.0 * (beta * log(x[i])));
-    ax[i] = -0.25 * X * d0;
-    cx[i] = -0.25 * X * d2;
-  }
-  for (int j = threadIdx.x; j < ny; j += blockDim.x) {
-    double d0 = 0.0, d2 = 0.0;
-    if (j >= 1 && j <= ny - 2) {
-      const double dl = y[j] - y[j - 1], du = y[j + 1] - y[j];
-      d0 = 2.0 / dl / (dl + du);
-      d2 = 2.0 / du / (dl + du);
-    }
-    ay[j] = -0.25 * nu * nu * d0;
-    cy[j] = -0.25 * nu * nu * d2;
-  }
-  T *res = R + pr.state_off;
-  const int64_t N = (int64_t)nx * ny;
-  for (int64_t e = threadIdx.x; e < N; e += blockDim.x) {
-    const double v = x[e % nx] - pr.strike;
-    res[e] = (T)(v > 0.0 ? v : 0.0);
-  }
-}
 
 // Timeline point k of initGrid (:14), t * k / (numT - 1): a multiply and a
 // divide, bit-identical to a C evaluation of the same expression.
@@ -116,18 +60,224 @@ __device__ __forceinline__ double timeline(double t, int k, int nt) {
   return t * (double)k / ((double)nt - 1.0);
 }
 
+// Reciprocals: v_rcp_f32 (1 ulp); v_rcp_f64 refined by two Newton steps.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      __float_as_int(old), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double old, double v) {
+  const long long o = __double_as_longlong(old), s = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)s, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(s >> 32), CTRL,
+                                             0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Lanes of one system: `seg` = this lane's index in its group of `G` lanes.
+struct Lanes {
+  int seg, G;
+  bool row;  // G <= 16: the group lies inside one DPP row
+};
+
+// Value of lane seg-st of the group (ident below the group's first lane).
+template <typename T>
+__device__ __forceinline__ T from_below(const Lanes &L, T v, int st, T ident) {
+  T r;
+  if (L.row) {
+    switch (st) {
+      case 1: r = dpp<0x111>(ident, v); break;  // row_shr:1
+      case 2: r = dpp<0x112>(ident, v); break;
+      case 4: r = dpp<0x114>(ident, v); break;
+      default: r = dpp<0x118>(ident, v); break;
+    }
+  } else {
+    r = __shfl_up(v, st, L.G);
+  }
+  return L.seg >= st ? r : ident;
+}
+
+// Value of lane seg+st of the group (ident past the group's last lane).
+template <typename T>
+__device__ __forceinline__ T from_above(const Lanes &L, T v, int st, T ident) {
+  T r;
+  if (L.row) {
+    switch (st) {
+      case 1: r = dpp<0x101>(ident, v); break;  // row_shl:1
+      case 2: r = dpp<0x102>(ident, v); break;
+      case 4: r = dpp<0x104>(ident, v); break;
+      default: r = dpp<0x108>(ident, v); break;
+    }
+  } else {
+    r = __shfl_down(v, st, L.G);
+  }
+  return L.seg + st < L.G ? r : ident;
+}
+
+// M consecutive values p[i0 .. i0+M-1] (0 past n); `vec` says p + i0 is 16-byte
+// aligned whenever the segment is complete.
+template <typename T, int M>
+__device__ __forceinline__ void load_seg(const T *__restrict__ p, int i0, int n,
+                                         bool vec, T (&o)[M]) {
+  if (vec && i0 + M <= n) {
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int q = 0; q < M / 4; ++q) {
+        const float4 v = *reinterpret_cast<const float4 *>(p + i0 + 4 * q);
+        o[4 * q] = v.x;
+        o[4 * q + 1] = v.y;
+        o[4 * q + 2] = v.z;
+        o[4 * q + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < M / 2; ++q) {
+        const double2 v = *reinterpret_cast<const double2 *>(p + i0 + 2 * q);
+        o[2 * q] = v.x;
+        o[2 * q + 1] = v.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; ++k) o[k] = (i0 + k < n) ? p[i0 + k] : T(0);
+  }
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void store_seg(T *__restrict__ p, int i0, int n,
+                                          bool vec, const T (&o)[M]) {
+  if (vec && i0 + M <= n) {
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int q = 0; q < M / 4; ++q)
+        *reinterpret_cast<float4 *>(p + i0 + 4 * q) =
+            make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < M / 2; ++q)
+        *reinterpret_cast<double2 *>(p + i0 + 2 * q) = make_double2(o[2 * q], o[2 * q + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      if (i0 + k < n) p[i0 + k] = o[k];
+  }
+}
+
+// Coefficient tables of the geometry buffer in the sweep's real type.
+template <typename T>
+struct Taps {
+  const T *ax, *cx, *ay, *cy;
+  __device__ Taps(const double *g, int nx, int ny) {
+    if constexpr (sizeof(T) == 4) {
+      const float *f = reinterpret_cast<const float *>(g + LVC_GEO_F64(nx, ny));
+      ax = f;
+      cx = f + LVC_P4(nx);
+      ay = cx + LVC_P4(nx);
+      cy = ay + LVC_P4(ny);
+    } else {
+      ax = g + nx;
+      cx = g + 2 * nx;
+      ay = g + 3 * nx + ny;
+      cy = g + 3 * nx + 2 * ny;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Setup: initGrid (:12-23), initOperator (:26-44) taps, the VarX factor
+// x^(2 beta) of updateParams (:61-65) and setPayoff (:46-47).  Computed in
+// FP64 with multiply-add contraction disabled, so the grid is bit-identical
+// to a C restatement of the same FP64 lines.  Grid: (problems, row chunks);
+// chunk 0 also writes the geometry and the descriptor fields.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
+                                                        double *geo, T *R) {
+#pragma clang fp contract(off)
+  lvc_problem &pr = probs[blockIdx.x];
+  const int nx = pr.nx, ny = pr.ny;
+  const double s0 = pr.s0, t = pr.t, alpha = pr.alpha, nu = pr.nu, beta = pr.beta;
+  const double stdX = 20.0 * alpha * s0 * sqrt(t);
+  const double dx = stdX / (double)nx;
+  const int xi = (int)(s0 / dx);  // i32.f64: truncation toward zero
+
+  // payoff rows of this chunk: R[j][i] = max(x_i - K, 0)
+  T *res = R + pr.state_off;
+  const int rows = (ny + gridDim.y - 1) / gridDim.y;
+  const int r0 = blockIdx.y * rows, r1 = min(ny, r0 + rows);
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+    const double xv = (double)i * dx - (double)xi * dx + s0;
+    const double v = xv - pr.strike;
+    const T pay = (T)(v > 0.0 ? v : 0.0);
+    for (int j = r0; j < r1; ++j) res[(int64_t)j * nx + i] = pay;
+  }
+  if (blockIdx.y != 0) return;
+
+  const double logAlpha = log(alpha);
+  const double stdY = 10.0 * nu * sqrt(t);
+  const double dy = stdY / (double)ny;
+  const int yi = ny / 2;
+  double *x = geo + pr.geo_off;
+  double *ax = x + nx, *cx = ax + nx, *y = cx + nx, *ay = y + ny, *cy = ay + ny;
+  float *f32 = reinterpret_cast<float *>(geo + pr.geo_off + LVC_GEO_F64(nx, ny));
+  float *axf = f32, *cxf = axf + LVC_P4(nx), *ayf = cxf + LVC_P4(nx);
+  float *cyf = ayf + LVC_P4(ny);
+  for (int i = threadIdx.x; i < nx; i += blockDim.x)
+    x[i] = (double)i * dx - (double)xi * dx + s0;
+  for (int j = threadIdx.x; j < ny; j += blockDim.x)
+    y[j] = (double)j * dy - (double)yi * dy + logAlpha;
+  if (threadIdx.x == 0) {
+    pr.xindex = xi;
+    pr.yindex = yi;
+    pr.k = pr.nt - 2;
+    pr.status = (xi >= 0 && xi < nx) ? 0 : 3;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < LVC_P4(nx); i += blockDim.x) {
+    double d0 = 0.0, d2 = 0.0;
+    if (i >= 1 && i <= nx - 2) {
+      const double dl = x[i] - x[i - 1], du = x[i + 1] - x[i];
+      d0 = 2.0 / dl / (dl + du);
+      d2 = 2.0 / du / (dl + du);
+    }
+    if (i < nx) {
+      const double X = exp(2.0 * (beta * log(x[i])));
+      ax[i] = -0.25 * X * d0;
+      cx[i] = -0.25 * X * d2;
+    }
+    axf[i] = i < nx ? (float)ax[i] : 0.0f;
+    cxf[i] = i < nx ? (float)cx[i] : 0.0f;
+  }
+  for (int j = threadIdx.x; j < LVC_P4(ny); j += blockDim.x) {
+    double d0 = 0.0, d2 = 0.0;
+    if (j >= 1 && j <= ny - 2) {
+      const double dl = y[j] - y[j - 1], du = y[j + 1] - y[j];
+      d0 = 2.0 / dl / (dl + du);
+      d2 = 2.0 / du / (dl + du);
+    }
+    if (j < ny) {
+      ay[j] = -0.25 * nu * nu * d0;
+      cy[j] = -0.25 * nu * nu * d2;
+    }
+    ayf[j] = j < ny ? (float)ay[j] : 0.0f;
+    cyf[j] = j < ny ? (float)cy[j] : 0.0f;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Segment solver shared by both sweeps.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T shfl_up_g(T v, int d, int width) {
-  return __shfl_up(v, d, width);
-}
-template <typename T>
-__device__ __forceinline__ T shfl_down_g(T v, int d, int width) {
-  return __shfl_down(v, d, width);
-}
-
 constexpr int kMaxLevels = 6;  // PCR levels: lanes per system <= 64
 
 template <typename T, int M>
@@ -140,14 +290,13 @@ struct SegSys {
   T aL, cL;  // taps of row M-1 (the lane's reduced row)
   T k1[kMaxLevels], k2[kMaxLevels];
   T rB;      // reciprocal of the reduced pivot after PCR
-  int seg, lanes, levels;
+  Lanes L;
 
   // a, c: taps of rows 0..M-1 of this segment; s: diagonal shift per row
   // (dtInv for rows of the system, 1 for padding rows past its end).
   __device__ __forceinline__ void prepare(const T (&a)[M], const T (&cc)[M],
-                                          const T (&s)[M], int seg_, int lanes_) {
-    seg = seg_;
-    lanes = lanes_;
+                                          const T (&s)[M], Lanes lanes) {
+    L = lanes;
 #pragma unroll
     for (int k = 0; k < M; ++k) c[k] = cc[k];
     aL = a[M - 1];
@@ -155,13 +304,13 @@ struct SegSys {
     // Local factorisation of rows 0..M-2: g = pivot + |c|, every update a sum
     // of positive terms for an M-matrix.
     T g = s[0] - a[0];
-    rb[0] = T(1) / (g - c[0]);
+    rb[0] = rcp(g - c[0]);
     m[0] = T(0);
 #pragma unroll
     for (int k = 1; k < M - 1; ++k) {
       m[k] = a[k] * rb[k - 1];
       g = s[k] - m[k] * g;
-      rb[k] = T(1) / (g - c[k]);
+      rb[k] = rcp(g - c[k]);
     }
     // Homogeneous responses v (to x[first-1]) and w (to x[last]).
     v[0] = -a[0];
@@ -175,45 +324,29 @@ struct SegSys {
       w[k] = -c[k] * w[k + 1] * rb[k];
     }
     // Reduced row of this lane: A X[seg-1] + B X[seg] + C X[seg+1] = D.
-    const bool last = (seg == lanes - 1);
-    T vn = shfl_down_g(v[0], 1, lanes), wn = shfl_down_g(w[0], 1, lanes);
-    if (last) {
-      vn = T(0);
-      wn = T(0);
-    }
+    const T vn = from_above(L, v[0], 1, T(0));
+    const T wn = from_above(L, w[0], 1, T(0));
     T A = aL * v[M - 2];
     T B = s[M - 1] - aL * (T(1) - w[M - 2]) - cL * (T(1) - vn);
     T C = cL * wn;
     // PCR on the matrix; the multipliers are kept for the right-hand sides.
-    levels = 0;
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
       const int st = 1 << l;
-      if (st < lanes) {
-        T Am = shfl_up_g(A, st, lanes), Bm = shfl_up_g(B, st, lanes);
-        T Cm = shfl_up_g(C, st, lanes);
-        T Ap = shfl_down_g(A, st, lanes), Bp = shfl_down_g(B, st, lanes);
-        T Cp = shfl_down_g(C, st, lanes);
-        if (seg < st) {
-          Am = T(0);
-          Bm = T(1);
-          Cm = T(0);
-        }
-        if (seg + st >= lanes) {
-          Ap = T(0);
-          Bp = T(1);
-          Cp = T(0);
-        }
-        const T q1 = A / Bm, q2 = C / Bp;
+      if (st < L.G) {
+        const T Am = from_below(L, A, st, T(0)), Bm = from_below(L, B, st, T(1));
+        const T Cm = from_below(L, C, st, T(0));
+        const T Ap = from_above(L, A, st, T(0)), Bp = from_above(L, B, st, T(1));
+        const T Cp = from_above(L, C, st, T(0));
+        const T q1 = A * rcp(Bm), q2 = C * rcp(Bp);
         k1[l] = q1;
         k2[l] = q2;
         A = -Am * q1;
         C = -Cp * q2;
         B = B - Cm * q1 - Ap * q2;
-        levels = l + 1;
       }
     }
-    rB = T(1) / B;
+    rB = rcp(B);
   }
 
   // Solves for one right-hand side d (rows 0..M-1 of this segment).
@@ -224,53 +357,40 @@ struct SegSys {
     x[M - 2] = x[M - 2] * rb[M - 2];
 #pragma unroll
     for (int k = M - 3; k >= 0; --k) x[k] = (x[k] - c[k] * x[k + 1]) * rb[k];
-    T un = shfl_down_g(x[0], 1, lanes);
-    if (seg == lanes - 1) un = T(0);
+    const T un = from_above(L, x[0], 1, T(0));
     T D = d[M - 1] - aL * x[M - 2] - cL * un;
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
-      if (l < levels) {
-        const int st = 1 << l;
-        T Dm = shfl_up_g(D, st, lanes), Dp = shfl_down_g(D, st, lanes);
-        if (seg < st) Dm = T(0);
-        if (seg + st >= lanes) Dp = T(0);
+      const int st = 1 << l;
+      if (st < L.G) {
+        const T Dm = from_below(L, D, st, T(0)), Dp = from_above(L, D, st, T(0));
         D = D - Dm * k1[l] - Dp * k2[l];
       }
     }
     const T X = D * rB;
-    T XL = shfl_up_g(X, 1, lanes);
-    if (seg == 0) XL = T(0);
+    const T XL = from_below(L, X, 1, T(0));
 #pragma unroll
     for (int k = 0; k < M - 1; ++k) x[k] = x[k] + v[k] * XL + w[k] * X;
     x[M - 1] = X;
   }
-
-  // Values of the neighbouring unknowns x[first-1] and x[last+1] (0 outside
-  // the system).
-  __device__ __forceinline__ void halo(const T (&x)[M], T &xl, T &xr) const {
-    xl = shfl_up_g(x[M - 1], 1, lanes);
-    xr = shfl_down_g(x[0], 1, lanes);
-    if (seg == 0) xl = T(0);
-    if (seg == lanes - 1) xr = T(0);
-  }
 };
 
-// One refinement pass: r = d - A x in FP64 with the FP64 taps, e = A^-1 r
-// with the FP32 factorisation, x += e in FP64.
+// One refinement pass on the solution x of  s*x + a*(x[-1]-x) + c*(x[+1]-x) = d:
+// the residual in FP64, a second solve with the same factorisation, the
+// correction added in FP64.
 template <typename T, int M>
 __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
-                                       const double (&a64)[M],
-                                       const double (&c64)[M],
-                                       const double (&s64)[M], T (&x)[M]) {
-  T xl, xr;
-  sys.halo(x, xl, xr);
+                                       const T (&a)[M], const T (&c)[M],
+                                       const T (&s)[M], T (&x)[M]) {
+  const T xl = from_below(sys.L, x[M - 1], 1, T(0));
+  const T xr = from_above(sys.L, x[0], 1, T(0));
   T r[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const double xk = (double)x[k];
     const double xm = (double)(k > 0 ? x[k - 1] : xl);
     const double xp = (double)(k < M - 1 ? x[k + 1] : xr);
-    const double ax = s64[k] * xk + a64[k] * (xm - xk) + c64[k] * (xp - xk);
+    const double ax = (double)s[k] * xk + (double)a[k] * (xm - xk) + (double)c[k] * (xp - xk);
     r[k] = (T)((double)d[k] - ax);
   }
   T e[M];
@@ -281,12 +401,19 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
 
 // ---------------------------------------------------------------------------
 // x sweep: one workgroup = `count` rows of one problem, `lanes` lanes per row.
+// U (the right-hand side) and V (the explicit y term) wait in LDS while the
+// solver's registers are live.
 // ---------------------------------------------------------------------------
+template <typename T, int M>
+constexpr int kXsweepLds = 2 * M * LVC_BLOCK;  // elements of T
+
 template <typename T, int M, bool REF>
-__global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
-    const lvc_problem *__restrict__ probs, const lvc_tile *__restrict__ tiles,
-    const double *__restrict__ geo, const T *__restrict__ R, T *__restrict__ Y) {
-  const lvc_tile tile = tiles[blockIdx.x];
+__device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
+                                            const lvc_problem *__restrict__ probs,
+                                            const double *__restrict__ geo,
+                                            const T *__restrict__ R,
+                                            T *__restrict__ Y, T *smem) {
+  T *lds_u = smem, *lds_v = smem + M * LVC_BLOCK;
   const lvc_problem &pr = probs[tile.problem];
   const int kt = pr.k;
   if (kt < 0) return;  // problem finished: whole workgroup exits
@@ -296,61 +423,69 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
   if (rloc >= tile.count) return;  // whole lane group exits together
   const int nx = pr.nx, ny = pr.ny;
   const int j = tile.first + rloc;
-  const double *gx = geo + pr.geo_off;
-  const double *ax = gx + nx, *cx = ax + nx, *gy = cx + nx;
-  const double *ay = gy + ny, *cy = ay + ny;
+  const double *g = geo + pr.geo_off;
+  const Taps<T> tp(g, nx, ny);
 
   const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
-  const double dt64 = 1.0 / (tnext - tnow);
-  const T dt = (T)dt64;
+  const T dt = (T)(1.0 / (tnext - tnow));
   const double nu = pr.nu;
-  const double E = exp(2.0 * gy[j] - nu * nu * tnow);  // exp(2y - nu^2 t)
-  const T ayj = (T)ay[j], cyj = (T)cy[j];
+  const T E = (T)exp(2.0 * g[3 * nx + j] - nu * nu * tnow);  // exp(2y - nu^2 t)
+  const T ayj = tp.ay[j], cyj = tp.cy[j];
+  const bool vec = (nx & 3) == 0;
 
   const T *row = R + pr.state_off + (int64_t)j * nx;
   const int i0 = seg * M;
-  T rc[M], rm[M], rp[M];
-  T rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
-  T rr = (i0 + M < nx) ? row[i0 + M] : T(0);
+  T rc[M];
+  load_seg<T, M>(row, i0, nx, vec, rc);
+  const T rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
+  const T rr = (i0 + M < nx) ? row[i0 + M] : T(0);
+  T a[M], c[M], s[M], U[M];
+  {
+    T rm[M], rp[M];
+    if (j > 0) load_seg<T, M>(row - nx, i0, nx, vec, rm);
+    if (j < ny - 1) load_seg<T, M>(row + nx, i0, nx, vec, rp);
+    load_seg<T, M>(tp.ax, i0, nx, vec, a);
+    load_seg<T, M>(tp.cx, i0, nx, vec, c);
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const int i = i0 + k;
-    const bool in = i < nx;
-    rc[k] = in ? row[i] : T(0);
-    rm[k] = (in && j > 0) ? row[i - nx] : T(0);
-    rp[k] = (in && j < ny - 1) ? row[i + nx] : T(0);
-  }
-  T a[M], c[M], s[M], U[M], V[M];
-  double a64[M], c64[M], s64[M];
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const int i = i0 + k;
-    const bool in = i < nx;
-    a64[k] = in ? ax[i] * E : 0.0;
-    c64[k] = in ? cx[i] * E : 0.0;
-    s64[k] = in ? (double)dt : 1.0;  // the time step the sweep uses
-    a[k] = (T)a64[k];
-    c[k] = (T)c64[k];
-    s[k] = in ? dt : T(1);
-    const T r0 = rc[k];
-    const T rw = k > 0 ? rc[k - 1] : rl;
-    const T re = k < M - 1 ? rc[k + 1] : rr;
-    const T ux = a[k] * (rw - r0) + c[k] * (re - r0);            // 0.5 * explicitX
-    const T vy = T(-2) * (ayj * (rm[k] - r0) + cyj * (rp[k] - r0));  // explicitY
-    V[k] = vy;
-    U[k] = dt * r0 - ux + vy;
+    for (int k = 0; k < M; ++k) {
+      const bool in = i0 + k < nx;
+      a[k] *= E;
+      c[k] *= E;
+      s[k] = in ? dt : T(1);
+      const T r0 = rc[k];
+      const T rw = k > 0 ? rc[k - 1] : rl;
+      const T re = k < M - 1 ? rc[k + 1] : rr;
+      const T dm = j > 0 ? rm[k] - r0 : T(0);
+      const T dp = j < ny - 1 ? rp[k] - r0 : T(0);
+      const T ux = a[k] * (rw - r0) + c[k] * (re - r0);  // 0.5 * explicitX
+      const T vy = T(-2) * (ayj * dm + cyj * dp);          // explicitY
+      lds_v[k * LVC_BLOCK + threadIdx.x] = vy;
+      U[k] = dt * r0 - ux + vy;
+      if (REF) lds_u[k * LVC_BLOCK + threadIdx.x] = U[k];
+    }
   }
   SegSys<T, M> sys;
-  sys.prepare(a, c, s, seg, G);
+  sys.prepare(a, c, s, Lanes{seg, G, G <= 16});
   T x[M];
   sys.solve(U, x);
-  if (REF) refine(sys, U, a64, c64, s64, x);
-  T *out = Y + pr.state_off + (int64_t)j * nx;
+  if (REF) {
+    T d[M];
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const int i = i0 + k;
-    if (i < nx) out[i] = dt * x[k] - T(0.5) * V[k];
+    for (int k = 0; k < M; ++k) d[k] = lds_u[k * LVC_BLOCK + threadIdx.x];
+    refine(sys, d, a, c, s, x);
   }
+  T out[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) out[k] = dt * x[k] - T(0.5) * lds_v[k * LVC_BLOCK + threadIdx.x];
+  store_seg<T, M>(Y + pr.state_off + (int64_t)j * nx, i0, nx, vec, out);
+}
+
+template <typename T, int M, bool REF>
+__global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
+    const lvc_problem *__restrict__ probs, const lvc_tile *__restrict__ tiles,
+    const double *__restrict__ geo, const T *__restrict__ R, T *__restrict__ Y) {
+  __shared__ T smem[kXsweepLds<T, M>];
+  xsweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, R, Y, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -359,13 +494,16 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
 // stores), stored segment-interleaved so that lanes of one column read
 // different banks.
 // ---------------------------------------------------------------------------
+template <typename T, int M>
+constexpr int kYsweepLds = M * (LVC_BLOCK + 64);  // elements of T
+
+// Every thread of the workgroup must call this (it holds barriers).
 template <typename T, int M, bool REF>
-__global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
-    lvc_problem *probs, const lvc_tile *__restrict__ tiles,
-    const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R) {
-  __shared__ T lds[M * (LVC_BLOCK + 64)];
-  const lvc_tile tile = tiles[blockIdx.x];
-  const lvc_problem &pr = probs[tile.problem];
+__device__ __forceinline__ void ysweep_tile(const lvc_tile tile, lvc_problem *probs,
+                                            const double *__restrict__ geo,
+                                            const T *__restrict__ Y,
+                                            T *__restrict__ R, T *lds) {
+  lvc_problem &pr = probs[tile.problem];
   const int kt = pr.k;
   if (kt < 0) return;
   const int G = tile.lanes;
@@ -385,33 +523,30 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
   }
   __syncthreads();
 
-  const double *gx = geo + pr.geo_off;
-  const double *gy = gx + 3 * nx, *ay = gy + ny, *cy = ay + ny;
+  const Taps<T> tp(geo + pr.geo_off, nx, ny);
   const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
-  const double dt64 = 1.0 / (tnext - tnow);
-  const T dt = (T)dt64;
+  const T dt = (T)(1.0 / (tnext - tnow));
   const int q = threadIdx.x & (G - 1);
   const int cc = threadIdx.x / G;
   const int j0 = q * M;
   T a[M], c[M], s[M], d[M];
-  double a64[M], c64[M], s64[M];
+  const bool vec = ((nx | ny) & 3) == 0;
+  load_seg<T, M>(tp.ay, j0, ny, vec, a);
+  load_seg<T, M>(tp.cy, j0, ny, vec, c);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    const int jj = j0 + k;
-    const bool in = jj < ny;
-    a64[k] = in ? ay[jj] : 0.0;
-    c64[k] = in ? cy[jj] : 0.0;
-    s64[k] = in ? (double)dt : 1.0;  // the time step the sweep uses
-    a[k] = (T)a64[k];
-    c[k] = (T)c64[k];
-    s[k] = in ? dt : T(1);
-    d[k] = lds[lidx(jj, cc)];
+    s[k] = j0 + k < ny ? dt : T(1);
+    d[k] = lds[lidx(j0 + k, cc)];
   }
   SegSys<T, M> sys;
-  sys.prepare(a, c, s, q, G);
+  sys.prepare(a, c, s, Lanes{q, G, G <= 16});
   T x[M];
   sys.solve(d, x);
-  if (REF) refine(sys, d, a64, c64, s64, x);
+  if (REF) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) d[k] = lds[lidx(j0 + k, cc)];
+    refine(sys, d, a, c, s, x);
+  }
   __syncthreads();  // every lane has read its panel values
 #pragma unroll
   for (int k = 0; k < M; ++k) lds[lidx(j0 + k, cc)] = x[k];
@@ -424,11 +559,90 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
   // The last of the problem's y workgroups to finish advances its time index
   // (every workgroup of the problem has read `k` before it counts itself).
   if (threadIdx.x == 0) {
-    lvc_problem *pw = const_cast<lvc_problem *>(&pr);
-    const int done = atomicAdd(&pw->ydone, 1) + 1;
-    if (done == pw->ytiles) {
-      pw->ydone = 0;
-      pw->k = kt - 1;
+    const int done = atomicAdd(&pr.ydone, 1) + 1;
+    if (done == pr.ytiles) {
+      pr.ydone = 0;
+      pr.xdone = 0;
+      pr.k = kt - 1;
+    }
+  }
+}
+
+template <typename T, int M, bool REF>
+__global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
+    lvc_problem *probs, const lvc_tile *__restrict__ tiles,
+    const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R) {
+  __shared__ T smem[kYsweepLds<T, M>];
+  ysweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, Y, R, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Fused step: a persistent launch over a work queue of the step's x tiles
+// followed by its y tiles.  A workgroup that draws a y tile of problem p
+// first waits until every x tile of p has been written: those tiles were
+// drawn earlier by running workgroups that never wait, so the wait ends
+// (it is bounded anyway: on expiry it raises ctl->error and goes on).  X
+// tiles publish with an agent-scope release, y tiles read after an
+// agent-scope acquire (the L2s of the XCDs are not coherent otherwise).
+// ---------------------------------------------------------------------------
+constexpr int kSpinLimit = 1 << 20;
+
+template <typename T, int M, bool REF>
+__global__ __launch_bounds__(LVC_BLOCK) void lvc_step_kernel(
+    lvc_problem *probs, const lvc_tile *__restrict__ xt,
+    const lvc_tile *__restrict__ yt, const int32_t *__restrict__ active,
+    lvc_ctl *ctl, const double *__restrict__ geo, T *R, T *Y) {
+  constexpr int kLds = kXsweepLds<T, M> > kYsweepLds<T, M> ? kXsweepLds<T, M>
+                                                            : kYsweepLds<T, M>;
+  __shared__ T smem[kLds];
+  __shared__ int s_tile;
+  const int step = __hip_atomic_load(&ctl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int nx_t = active[2 * step], ny_t = active[2 * step + 1];
+  // after an expired wait every later launch only counts itself out
+  const bool failed =
+      __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+  for (; !failed;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->next, 1);
+    __syncthreads();
+    const int t = s_tile;
+    __syncthreads();  // s_tile is read before it is redrawn
+    if (t >= nx_t + ny_t) break;
+    if (t < nx_t) {
+      const lvc_tile tile = xt[t];
+      xsweep_tile<T, M, REF>(tile, probs, geo, R, Y, smem);
+      __threadfence();  // this thread's Y stores, agent scope
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(&probs[tile.problem].xdone, 1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const lvc_tile tile = yt[t - nx_t];
+      if (threadIdx.x == 0) {
+        lvc_problem &pr = probs[tile.problem];
+        int spins = 0;
+        while (__hip_atomic_load(&pr.xdone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+               pr.xtiles) {
+          if (++spins > kSpinLimit ||
+              __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            atomicOr(&ctl->error, 1);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      __syncthreads();
+      __threadfence();  // acquire side for every thread's Y loads
+      ysweep_tile<T, M, REF>(tile, probs, geo, Y, R, smem);
+      __syncthreads();  // LDS reuse by the next tile
+    }
+  }
+  // the last workgroup out resets the queue and moves to the next step
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(&ctl->done, 1) + 1;
+    if (done == (int)gridDim.x) {
+      ctl->next = 0;
+      ctl->done = 0;
+      ctl->step = step + 1;
     }
   }
 }
@@ -459,6 +673,28 @@ int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
     lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, 0, st>>>(
         const_cast<lvc_problem *>(probs), tiles, geo, (const T *)in, (T *)out);
   return (int)hipGetLastError();
+}
+
+template <typename T, int M, bool REF>
+int launch_step_t(hipStream_t st, lvc_problem *probs, const lvc_tile *xt,
+                  const lvc_tile *yt, const int32_t *active, lvc_ctl *ctl,
+                  int grid, const double *geo, void *R, void *Y) {
+  if (grid <= 0) return 0;
+  lvc_step_kernel<T, M, REF><<<grid, LVC_BLOCK, 0, st>>>(probs, xt, yt, active, ctl,
+                                                         geo, (T *)R, (T *)Y);
+  return (int)hipGetLastError();
+}
+
+template <typename T, bool REF>
+int dispatch_step(hipStream_t st, int seg, lvc_problem *probs, const lvc_tile *xt,
+                  const lvc_tile *yt, const int32_t *active, lvc_ctl *ctl, int grid,
+                  const double *geo, void *R, void *Y) {
+  switch (seg) {
+    case 4: return launch_step_t<T, 4, REF>(st, probs, xt, yt, active, ctl, grid, geo, R, Y);
+    case 8: return launch_step_t<T, 8, REF>(st, probs, xt, yt, active, ctl, grid, geo, R, Y);
+    case 16: return launch_step_t<T, 16, REF>(st, probs, xt, yt, active, ctl, grid, geo, R, Y);
+    default: return (int)hipErrorInvalidValue;
+  }
 }
 
 template <typename T, bool REF>
@@ -494,10 +730,11 @@ int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
                      double *geo, void *R) {
   if (nprob <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(nprob, 8);
   if (precision == LVC_FP64)
-    lvc_setup_kernel<double><<<nprob, 256, 0, st>>>(probs, geo, (double *)R);
+    lvc_setup_kernel<double><<<grid, 256, 0, st>>>(probs, geo, (double *)R);
   else
-    lvc_setup_kernel<float><<<nprob, 256, 0, st>>>(probs, geo, (float *)R);
+    lvc_setup_kernel<float><<<grid, 256, 0, st>>>(probs, geo, (float *)R);
   return (int)hipGetLastError();
 }
 
@@ -513,6 +750,20 @@ int lvc_launch_ysweep(void *stream, int precision, int seg,
                       int ntiles, const double *geo, const void *Y, void *R) {
   return dispatch((hipStream_t)stream, false, precision, seg, probs, tiles,
                   ntiles, geo, Y, R);
+}
+
+int lvc_launch_step(void *stream, int precision, int seg_x, int seg_y,
+                    lvc_problem *probs, const lvc_tile *xtiles,
+                    const lvc_tile *ytiles, const int32_t *active, lvc_ctl *ctl,
+                    int grid, const double *geo, void *R, void *Y) {
+  if (seg_x != seg_y) return (int)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  switch (precision) {
+    case LVC_FP32: return dispatch_step<float, false>(st, seg_x, probs, xtiles, ytiles, active, ctl, grid, geo, R, Y);
+    case LVC_FP32_REFINE: return dispatch_step<float, true>(st, seg_x, probs, xtiles, ytiles, active, ctl, grid, geo, R, Y);
+    case LVC_FP64: return dispatch_step<double, false>(st, seg_x, probs, xtiles, ytiles, active, ctl, grid, geo, R, Y);
+    default: return (int)hipErrorInvalidValue;
+  }
 }
 
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,

@@ -51,9 +51,10 @@ int lvc_ready(struct futhark_context *ctx) {
 
 static const char *const tuning_names[] = {
     "main.suff_intra_par_0", "main.suff_intra_par_1", "lvc.precision",
-    "lvc.seg_x", "lvc.seg_y", "lvc.autotune"};
+    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.fused_step", "lvc.graph"};
 static const char *const tuning_classes[] = {
-    "threshold", "threshold", "precision", "segment", "segment", "flag"};
+    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag",
+    "flag"};
 #define NTUNING ((int)(sizeof(tuning_names) / sizeof(tuning_names[0])))
 
 int futhark_get_tuning_param_count(void) { return NTUNING; }
@@ -70,6 +71,8 @@ struct futhark_context_config *futhark_context_config_new(void) {
     return NULL;
   cfg->device = 0;
   cfg->precision = LVC_FP32_REFINE;
+  cfg->fused = 1;
+  cfg->graph = 1;
   cfg->suff_intra_par_0 = 32; /* finpar/LocVolCalib.fut.tuning:1-2 */
   cfg->suff_intra_par_1 = 32;
   return cfg;
@@ -129,6 +132,14 @@ int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
       cfg->seg_y = (int)v;
     return 0;
   }
+  if (strcmp(name, "lvc.fused_step") == 0) {
+    cfg->fused = v != 0;
+    return 0;
+  }
+  if (strcmp(name, "lvc.graph") == 0) {
+    cfg->graph = v != 0;
+    return 0;
+  }
   if (strcmp(name, "lvc.autotune") == 0) {
     cfg->autotune = v != 0;
     return 0;
@@ -182,6 +193,9 @@ struct futhark_context *futhark_context_new(struct futhark_context_config *cfg) 
                                                  hipEventDisableTiming),
                     "hipEventCreate"))
     return ctx;
+  if (hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount,
+                            ctx->device) != hipSuccess || ctx->num_cus <= 0)
+    ctx->num_cus = 256;
   ctx->ok = 1;
   lvc_tune_load(ctx);
   return ctx;

@@ -22,6 +22,8 @@ struct futhark_context_config {
   int precision; /* enum lvc_precision for futhark_entry_main */
   int seg_x, seg_y; /* 0 = tuned */
   int autotune;
+  int fused; /* 1: one persistent launch per step; 0: x and y sweep launches */
+  int graph; /* replay the step launches from a HIP graph */
   size_t suff_intra_par_0, suff_intra_par_1;
 };
 
@@ -74,6 +76,7 @@ struct futhark_context {
   int nevents, capevents;
   /* tuning table */
   lvc_tuned tuned[LVC_MAX_TUNED];
+  int num_cus; /* compute units of the device */
   int ntuned;
 };
 
@@ -112,9 +115,15 @@ typedef struct lvc_batch {
   size_t geo_bytes;
   void *d_R, *d_Y;
   size_t state_bytes;
-  lvc_tile *d_tiles; /* x tiles then y tiles */
+  lvc_tile *d_tiles; /* x tiles then y tiles, each by problem numT, longest first */
   size_t tiles_bytes;
   int nxt, nyt;
+  /* tiles of the problems still running at step s: a prefix of each list */
+  int *x_active, *y_active; /* [steps] */
+  /* fused step */
+  int fused, graph, grid;
+  int32_t *d_active; /* [steps][2] */
+  lvc_ctl *d_ctl;
 } lvc_batch;
 
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,

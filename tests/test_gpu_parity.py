@@ -255,7 +255,8 @@ def test_server_alone_vs_mixed():
 def test_server_memory_budget():
     say("case: server memory budget")
     with lvc.LocVolCalib() as ctx:
-        one = 2 * 64 * 64 * 4 + 8 * (6 * 64)
+        # two f32 grids + geometry (FP64 x, taps; FP32 taps) of a 64x64 request
+        one = 2 * 64 * 64 * 4 + 8 * (6 * 64 + 2 * 64)
         srv = lvc.Server(ctx, max_slots=8, mem_budget=2 * one + 10)
         try:
             ids = [srv.submit(64, 64, 10, 0.03, 0.001 * i, 5.0, 0.2, 0.6, 0.5)
