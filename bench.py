@@ -80,12 +80,88 @@ def cpu_baseline(budget_s=12.0):
                       f"tridagPar as LocVolCalib.fut:68-124), OpenMP over options, {el:.1f} s"}
 
 
+def uneven_problems(n, seed, sizes=(32, 64, 128, 256), nt=(32, 256)):
+    """Seeded uneven problems: numX, numY from `sizes`, numT uniform in `nt`,
+    s0 U[0.02,0.05], K U[0,0.1], t U[1,10], alpha U[0.1,0.3], nu U[0.4,0.8],
+    beta U[0.3,0.7]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return dict(numX=rng.choice(sizes, n).astype(np.int32),
+                numY=rng.choice(sizes, n).astype(np.int32),
+                numT=rng.integers(nt[0], nt[1] + 1, n).astype(np.int32),
+                s0=rng.uniform(0.02, 0.05, n), strike=rng.uniform(0.0, 0.1, n),
+                t=rng.uniform(1, 10, n), alpha=rng.uniform(0.1, 0.3, n),
+                nu=rng.uniform(0.4, 0.8, n), beta=rng.uniform(0.3, 0.7, n))
+
+
+def bench_batched(lvc, reps=3, n=512, seed=42):
+    """512 uneven problems in one batch call (FP32 + refinement)."""
+    pr = uneven_problems(n, seed)
+    with lvc.LocVolCalib(device=0) as ctx:
+        ctx.main_batch(**pr)                       # warm-up (and tuning)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.main_batch(**pr)
+        dt = (time.perf_counter() - t0) / reps
+    elems = float((pr["numX"].astype(float) * pr["numY"] * (pr["numT"] - 1)).sum())
+    return {"problems": n, "seed": seed, "ms_per_batch": round(dt * 1e3, 3),
+            "problems_per_s": round(n / dt, 2),
+            "grid_point_steps_per_s": round(elems / dt, 1)}
+
+
+def bench_serving(lvc, rate, duration_s=1.5, seed=7, max_slots=1024, budget=64 << 30):
+    """Poisson arrivals at `rate` req/s of single problems (sizes and
+    parameters as the batched set), continuous batching, FP32 + refinement.
+    Latency = completion - arrival (host clock)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * duration_s))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    pr = uneven_problems(n, seed + 1)
+    keys = ("numX", "numY", "numT", "s0", "strike", "t", "alpha", "nu", "beta")
+    reqs = [tuple(int(pr[k][i]) if k.startswith("num") else float(pr[k][i]) for k in keys)
+            for i in range(n)]
+    with lvc.LocVolCalib(device=0) as ctx:
+        srv = lvc.Server(ctx, max_slots=max_slots, mem_budget=budget)
+        warm = srv.submit(*reqs[0])
+        srv.drain()
+        ids, done_at = [None] * n, [None] * n
+        pending = set()
+        nxt = 0
+        t0 = time.perf_counter()
+        while nxt < n or pending:
+            now = time.perf_counter() - t0
+            while nxt < n and arrivals[nxt] <= now:
+                ids[nxt] = srv.submit(*reqs[nxt])
+                pending.add(nxt)
+                nxt += 1
+            if not pending:
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.001)))
+                continue
+            if srv.step() > 0:
+                now = time.perf_counter() - t0
+                for i in list(pending):
+                    if srv.poll(ids[i]) is not None:
+                        done_at[i] = now
+                        pending.discard(i)
+        total = time.perf_counter() - t0
+        srv.free()
+    lat = (np.array(done_at) - arrivals) * 1e3
+    return {"rate_req_per_s": rate, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "options_per_s": round(n / total, 2), "wall_s": round(total, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched and serving measurements")
+    ap.add_argument("--serve-rates", default="250,1000,5000",
+                    help="Poisson arrival rates (req/s) of the serving runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB (read) + 256 MiB "
                          "(write) three times, to scale HBM traffic counters")
@@ -110,15 +186,15 @@ def main():
         del a, b
 
     with lvc.LocVolCalib(device=0, profiling=True) as ctx:
-        # FP32 + refinement (headline)
+        # FP32 + refinement (headline): timed calls without per-launch events
         ctx.pause_profiling()
-        total32, lat32 = None, None
-        for _ in range(args.warmup):
-            ctx.free_f32(ctx.main_raw(OUTER, NX, NY, NT, *PARAMS))
-        ctx.sync()
+        total32, lat32 = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps,
+                                    args.warmup)
+        # the same calls again with HIP events around every launch on the
+        # library's stream: per-kernel durations for the roofline
         ctx.report()  # drain
         ctx.unpause_profiling()
-        total32, lat32 = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps, 0)
+        total32p, _ = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps, 0)
         rep32 = json.loads(ctx.report())
         ctx.pause_profiling()
         # FP64
@@ -165,11 +241,15 @@ def main():
         "max_rel_err_fp32refine_vs_fp64": rel,
         "kernels_us_per_launch": {k: round(v["total_us"] / max(v["count"], 1), 3)
                                   for k, v in rep32.items()},
+        "ms_per_step_with_events": round(1e3 * total32p / args.steps, 4),
         "roofline": {"bound": "hbm", "kernel": dom_name,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3)},
     }
+    if not args.no_forms:
+        out["batched"] = bench_batched(lvc)
+        out["serving"] = [bench_serving(lvc, float(r)) for r in args.serve_rates.split(",")]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

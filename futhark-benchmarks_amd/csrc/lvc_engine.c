@@ -104,8 +104,6 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     q.status = 0;
     q.ydone = 0;
     q.ytiles = (q.nx + cols - 1) / cols;
-    q.xtiles = (q.ny + rows - 1) / rows;
-    q.xdone = 0;
     geo += align64(LVC_GEO_LEN(q.nx, q.ny));
     state += align64((int64_t)q.nx * q.ny);
     nxt += (q.ny + rows - 1) / rows;
@@ -172,34 +170,7 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     rc = lvc_upload(ctx, b->d_probs, b->h_probs, b->probs_bytes) ||
          lvc_upload(ctx, b->d_tiles, tiles, b->tiles_bytes);
   free(tiles);
-  /* fused step: one persistent launch per step needs equal segment lengths */
-  b->fused = ctx->cfg && ctx->cfg->fused && seg_x == seg_y && steps > 0;
   b->graph = ctx->cfg && ctx->cfg->graph;
-  if (!rc && b->fused) {
-    int32_t *act = malloc((size_t)steps * 2 * sizeof(int32_t));
-    if (!act) {
-      lvc_set_error(ctx, "host allocation failed");
-      return 1;
-    }
-    int most = 0;
-    for (int64_t st = 0; st < steps; st++) {
-      act[2 * st] = b->x_active[st];
-      act[2 * st + 1] = b->y_active[st];
-      if (act[2 * st] + act[2 * st + 1] > most)
-        most = act[2 * st] + act[2 * st + 1];
-    }
-    /* persistent grid: what the device holds at once, at most one workgroup
-     * per tile */
-    const int cap = 4 * (ctx->num_cus > 0 ? ctx->num_cus : 256);
-    b->grid = most < cap ? most : cap;
-    const lvc_ctl zero = {0, 0, 0, 0};
-    rc = lvc_dev_alloc(ctx, (size_t)steps * 2 * sizeof(int32_t), (void **)&b->d_active) ||
-         lvc_dev_alloc(ctx, sizeof(lvc_ctl), (void **)&b->d_ctl);
-    if (!rc)
-      rc = lvc_upload(ctx, b->d_active, act, (size_t)steps * 2 * sizeof(int32_t)) ||
-           lvc_upload(ctx, b->d_ctl, &zero, sizeof(zero));
-    free(act);
-  }
   return rc;
 }
 
@@ -209,69 +180,44 @@ void lvc_batch_release(struct futhark_context *ctx, lvc_batch *b) {
   lvc_dev_free(ctx, b->d_R, b->state_bytes);
   lvc_dev_free(ctx, b->d_Y, b->state_bytes);
   lvc_dev_free(ctx, b->d_tiles, b->tiles_bytes);
-  if (b->d_active)
-    lvc_dev_free(ctx, b->d_active, (size_t)b->steps * 2 * sizeof(int32_t));
-  if (b->d_ctl)
-    lvc_dev_free(ctx, b->d_ctl, sizeof(lvc_ctl));
   free(b->h_probs);
   free(b->x_active);
   free(b->y_active);
   memset(b, 0, sizeof(*b));
 }
 
-/* Fused path: `steps` launches of the persistent step kernel, replayed from
- * a one-node HIP graph when enabled (the launch reads its step index from
- * the control block, so every launch is the same).  A bounded wait that ran
- * out inside the kernel is reported here. */
-static int run_fused(struct futhark_context *ctx, lvc_batch *b,
+/* The step sequence (an x and a y launch per time step, each over the tiles
+ * of the problems still running) captured into one HIP graph and launched
+ * once.  Synchronous: the graph is destroyed when it has run. */
+static int run_graph(struct futhark_context *ctx, lvc_batch *b,
                      const lvc_tile *xt, const lvc_tile *yt) {
   void *st = (void *)ctx->stream;
   hipGraph_t graph = NULL;
   hipGraphExec_t exec = NULL;
-  int rc = 0;
-  if (b->graph) {
-    rc = lvc_check_hip(ctx, hipStreamBeginCapture(ctx->stream,
-                                                  hipStreamCaptureModeThreadLocal),
-                       "hipStreamBeginCapture");
-    if (!rc) {
-      int e = lvc_launch_step(st, b->precision, b->seg_x, b->seg_y, b->d_probs, xt,
-                              yt, b->d_active, b->d_ctl, b->grid, b->d_geo, b->d_R,
-                              b->d_Y);
-      int e2 = (int)hipStreamEndCapture(ctx->stream, &graph);
-      rc = lvc_check_hip(ctx, e, "step kernel (capture)") ||
-           lvc_check_hip(ctx, e2, "hipStreamEndCapture") ||
-           lvc_check_hip(ctx, hipGraphInstantiate(&exec, graph, NULL, NULL, 0),
-                         "hipGraphInstantiate");
-    }
+  int rc = lvc_check_hip(ctx, hipStreamBeginCapture(ctx->stream,
+                                                    hipStreamCaptureModeThreadLocal),
+                         "hipStreamBeginCapture");
+  if (rc)
+    return rc;
+  int e = 0;
+  for (int64_t s = 0; !e && s < b->steps; s++) {
+    e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
+                          b->x_active[s], b->d_geo, b->d_R, b->d_Y);
+    if (!e)
+      e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
+                            b->y_active[s], b->d_geo, b->d_Y, b->d_R);
   }
-  for (int64_t s = 0; !rc && s < b->steps; s++) {
-    int tok = lvc_prof_begin(ctx, "lvc_step");
-    int e = exec ? (int)hipGraphLaunch(exec, ctx->stream)
-                 : lvc_launch_step(st, b->precision, b->seg_x, b->seg_y, b->d_probs,
-                                   xt, yt, b->d_active, b->d_ctl, b->grid, b->d_geo,
-                                   b->d_R, b->d_Y);
-    lvc_prof_end(ctx, tok);
-    rc = lvc_check_hip(ctx, e, "step kernel");
-  }
-  lvc_ctl ctl = {0, 0, 0, 0};
-  if (!rc)
-    rc = lvc_check_hip(ctx, hipMemcpyAsync(&ctl, b->d_ctl, sizeof(ctl),
-                                           hipMemcpyDeviceToHost, ctx->stream),
-                       "hipMemcpyAsync") ||
-         lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  const int e2 = (int)hipStreamEndCapture(ctx->stream, &graph);
+  rc = lvc_check_hip(ctx, e, "sweep kernel (capture)") ||
+       lvc_check_hip(ctx, e2, "hipStreamEndCapture") ||
+       lvc_check_hip(ctx, hipGraphInstantiate(&exec, graph, NULL, NULL, 0),
+                     "hipGraphInstantiate") ||
+       lvc_check_hip(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch") ||
+       lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   if (exec)
     (void)hipGraphExecDestroy(exec);
   if (graph)
     (void)hipGraphDestroy(graph);
-  if (!rc && ctl.error) {
-    lvc_set_error(ctx, "fused step: a wait for x tiles expired");
-    rc = 1;
-  }
-  if (!rc && ctl.step != b->steps) {
-    lvc_set_error(ctx, "fused step: %d of %lld steps ran", ctl.step,
-                  (long long)b->steps);
-    rc = 1;
-  }
   return rc;
 }
 
@@ -285,8 +231,9 @@ int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
   if (lvc_check_hip(ctx, e, "setup kernel"))
     return 1;
   const lvc_tile *xt = b->d_tiles, *yt = b->d_tiles + b->nxt;
-  if (b->fused)
-    return run_fused(ctx, b, xt, yt);
+  const int profiling = ctx->cfg && ctx->cfg->profiling && !ctx->prof_paused;
+  if (b->graph && !profiling)
+    return run_graph(ctx, b, xt, yt);
   for (int64_t s = 0; s < b->steps; s++) {
     tok = lvc_prof_begin(ctx, "lvc_xsweep");
     e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,

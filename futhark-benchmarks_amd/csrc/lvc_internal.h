@@ -41,9 +41,7 @@ typedef struct lvc_problem {
   int32_t status;          /* 0 = ok, 3 = price index out of range */
   int32_t ytiles;          /* y-sweep workgroups of this problem per step */
   int32_t ydone;           /* y-sweep workgroups finished in this step */
-  int32_t xtiles;          /* x-sweep tiles of this problem per step */
-  int32_t xdone;           /* x-sweep tiles finished in this step (fused step) */
-  int32_t pad_[2];
+  int32_t pad_[4];
   int64_t state_off;       /* element offset of R and Y */
   int64_t geo_off;         /* element offset into the f64 geometry buffer */
   double s0, strike, t, alpha, nu, beta;
@@ -58,17 +56,6 @@ typedef struct lvc_tile {
   int32_t count;
   int32_t lanes;
 } lvc_tile;
-
-/* Control block of the fused step kernel (device memory, zero-initialised):
- * a work queue over the step's x tiles then y tiles, a count of finished
- * workgroups (the last one resets the queue), the step counter that indexes
- * `active`, and an error flag (a bounded wait that ran out). */
-typedef struct lvc_ctl {
-  int32_t next;
-  int32_t done;
-  int32_t step;
-  int32_t error;
-} lvc_ctl;
 
 #define LVC_P4(n) (((int64_t)(n) + 3) & ~(int64_t)3)
 #define LVC_GEO_F64(nx, ny) ((3 * (int64_t)(nx) + 3 * (int64_t)(ny) + 1) & ~(int64_t)1)
@@ -88,15 +75,6 @@ int lvc_launch_xsweep(void *stream, int precision, int seg,
 int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *Y, void *R);
-/* Fused step: one persistent launch does the x sweep and the y sweep of
- * every running problem for one time step.  `active` holds, per step, the
- * number of running x tiles and y tiles (prefixes of the two lists);
- * ctl->step selects the entry and advances after each launch, so the same
- * launch can be replayed from a graph.  `grid` workgroups. */
-int lvc_launch_step(void *stream, int precision, int seg_x, int seg_y,
-                    lvc_problem *probs, const lvc_tile *xtiles,
-                    const lvc_tile *ytiles, const int32_t *active, lvc_ctl *ctl,
-                    int grid, const double *geo, void *R, void *Y);
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
                       int nprob, const void *R, void *out, int out_f64);
 

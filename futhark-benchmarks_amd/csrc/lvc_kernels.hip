@@ -562,7 +562,6 @@ __device__ __forceinline__ void ysweep_tile(const lvc_tile tile, lvc_problem *pr
     const int done = atomicAdd(&pr.ydone, 1) + 1;
     if (done == pr.ytiles) {
       pr.ydone = 0;
-      pr.xdone = 0;
       pr.k = kt - 1;
     }
   }
@@ -574,77 +573,6 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
     const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R) {
   __shared__ T smem[kYsweepLds<T, M>];
   ysweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, Y, R, smem);
-}
-
-// ---------------------------------------------------------------------------
-// Fused step: a persistent launch over a work queue of the step's x tiles
-// followed by its y tiles.  A workgroup that draws a y tile of problem p
-// first waits until every x tile of p has been written: those tiles were
-// drawn earlier by running workgroups that never wait, so the wait ends
-// (it is bounded anyway: on expiry it raises ctl->error and goes on).  X
-// tiles publish with an agent-scope release, y tiles read after an
-// agent-scope acquire (the L2s of the XCDs are not coherent otherwise).
-// ---------------------------------------------------------------------------
-constexpr int kSpinLimit = 1 << 20;
-
-template <typename T, int M, bool REF>
-__global__ __launch_bounds__(LVC_BLOCK) void lvc_step_kernel(
-    lvc_problem *probs, const lvc_tile *__restrict__ xt,
-    const lvc_tile *__restrict__ yt, const int32_t *__restrict__ active,
-    lvc_ctl *ctl, const double *__restrict__ geo, T *R, T *Y) {
-  constexpr int kLds = kXsweepLds<T, M> > kYsweepLds<T, M> ? kXsweepLds<T, M>
-                                                            : kYsweepLds<T, M>;
-  __shared__ T smem[kLds];
-  __shared__ int s_tile;
-  const int step = __hip_atomic_load(&ctl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int nx_t = active[2 * step], ny_t = active[2 * step + 1];
-  // after an expired wait every later launch only counts itself out
-  const bool failed =
-      __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-  for (; !failed;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->next, 1);
-    __syncthreads();
-    const int t = s_tile;
-    __syncthreads();  // s_tile is read before it is redrawn
-    if (t >= nx_t + ny_t) break;
-    if (t < nx_t) {
-      const lvc_tile tile = xt[t];
-      xsweep_tile<T, M, REF>(tile, probs, geo, R, Y, smem);
-      __threadfence();  // this thread's Y stores, agent scope
-      __syncthreads();
-      if (threadIdx.x == 0)
-        __hip_atomic_fetch_add(&probs[tile.problem].xdone, 1, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const lvc_tile tile = yt[t - nx_t];
-      if (threadIdx.x == 0) {
-        lvc_problem &pr = probs[tile.problem];
-        int spins = 0;
-        while (__hip_atomic_load(&pr.xdone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
-               pr.xtiles) {
-          if (++spins > kSpinLimit ||
-              __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            atomicOr(&ctl->error, 1);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-      }
-      __syncthreads();
-      __threadfence();  // acquire side for every thread's Y loads
-      ysweep_tile<T, M, REF>(tile, probs, geo, Y, R, smem);
-      __syncthreads();  // LDS reuse by the next tile
-    }
-  }
-  // the last workgroup out resets the queue and moves to the next step
-  if (threadIdx.x == 0) {
-    const int done = atomicAdd(&ctl->done, 1) + 1;
-    if (done == (int)gridDim.x) {
-      ctl->next = 0;
-      ctl->done = 0;
-      ctl->step = step + 1;
-    }
-  }
 }
 
 // value (:236): the price at (yindex, xindex).
@@ -673,28 +601,6 @@ int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
     lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, 0, st>>>(
         const_cast<lvc_problem *>(probs), tiles, geo, (const T *)in, (T *)out);
   return (int)hipGetLastError();
-}
-
-template <typename T, int M, bool REF>
-int launch_step_t(hipStream_t st, lvc_problem *probs, const lvc_tile *xt,
-                  const lvc_tile *yt, const int32_t *active, lvc_ctl *ctl,
-                  int grid, const double *geo, void *R, void *Y) {
-  if (grid <= 0) return 0;
-  lvc_step_kernel<T, M, REF><<<grid, LVC_BLOCK, 0, st>>>(probs, xt, yt, active, ctl,
-                                                         geo, (T *)R, (T *)Y);
-  return (int)hipGetLastError();
-}
-
-template <typename T, bool REF>
-int dispatch_step(hipStream_t st, int seg, lvc_problem *probs, const lvc_tile *xt,
-                  const lvc_tile *yt, const int32_t *active, lvc_ctl *ctl, int grid,
-                  const double *geo, void *R, void *Y) {
-  switch (seg) {
-    case 4: return launch_step_t<T, 4, REF>(st, probs, xt, yt, active, ctl, grid, geo, R, Y);
-    case 8: return launch_step_t<T, 8, REF>(st, probs, xt, yt, active, ctl, grid, geo, R, Y);
-    case 16: return launch_step_t<T, 16, REF>(st, probs, xt, yt, active, ctl, grid, geo, R, Y);
-    default: return (int)hipErrorInvalidValue;
-  }
 }
 
 template <typename T, bool REF>
@@ -750,20 +656,6 @@ int lvc_launch_ysweep(void *stream, int precision, int seg,
                       int ntiles, const double *geo, const void *Y, void *R) {
   return dispatch((hipStream_t)stream, false, precision, seg, probs, tiles,
                   ntiles, geo, Y, R);
-}
-
-int lvc_launch_step(void *stream, int precision, int seg_x, int seg_y,
-                    lvc_problem *probs, const lvc_tile *xtiles,
-                    const lvc_tile *ytiles, const int32_t *active, lvc_ctl *ctl,
-                    int grid, const double *geo, void *R, void *Y) {
-  if (seg_x != seg_y) return (int)hipErrorInvalidValue;
-  hipStream_t st = (hipStream_t)stream;
-  switch (precision) {
-    case LVC_FP32: return dispatch_step<float, false>(st, seg_x, probs, xtiles, ytiles, active, ctl, grid, geo, R, Y);
-    case LVC_FP32_REFINE: return dispatch_step<float, true>(st, seg_x, probs, xtiles, ytiles, active, ctl, grid, geo, R, Y);
-    case LVC_FP64: return dispatch_step<double, false>(st, seg_x, probs, xtiles, ytiles, active, ctl, grid, geo, R, Y);
-    default: return (int)hipErrorInvalidValue;
-  }
 }
 
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,

@@ -51,10 +51,9 @@ int lvc_ready(struct futhark_context *ctx) {
 
 static const char *const tuning_names[] = {
     "main.suff_intra_par_0", "main.suff_intra_par_1", "lvc.precision",
-    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.fused_step", "lvc.graph"};
+    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.graph"};
 static const char *const tuning_classes[] = {
-    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag",
-    "flag"};
+    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag"};
 #define NTUNING ((int)(sizeof(tuning_names) / sizeof(tuning_names[0])))
 
 int futhark_get_tuning_param_count(void) { return NTUNING; }
@@ -71,8 +70,7 @@ struct futhark_context_config *futhark_context_config_new(void) {
     return NULL;
   cfg->device = 0;
   cfg->precision = LVC_FP32_REFINE;
-  cfg->fused = 1;
-  cfg->graph = 1;
+  cfg->graph = 0;
   cfg->suff_intra_par_0 = 32; /* finpar/LocVolCalib.fut.tuning:1-2 */
   cfg->suff_intra_par_1 = 32;
   return cfg;
@@ -130,10 +128,6 @@ int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
       cfg->seg_x = (int)v;
     else
       cfg->seg_y = (int)v;
-    return 0;
-  }
-  if (strcmp(name, "lvc.fused_step") == 0) {
-    cfg->fused = v != 0;
     return 0;
   }
   if (strcmp(name, "lvc.graph") == 0) {

@@ -22,7 +22,6 @@ struct futhark_context_config {
   int precision; /* enum lvc_precision for futhark_entry_main */
   int seg_x, seg_y; /* 0 = tuned */
   int autotune;
-  int fused; /* 1: one persistent launch per step; 0: x and y sweep launches */
   int graph; /* replay the step launches from a HIP graph */
   size_t suff_intra_par_0, suff_intra_par_1;
 };
@@ -120,10 +119,7 @@ typedef struct lvc_batch {
   int nxt, nyt;
   /* tiles of the problems still running at step s: a prefix of each list */
   int *x_active, *y_active; /* [steps] */
-  /* fused step */
-  int fused, graph, grid;
-  int32_t *d_active; /* [steps][2] */
-  lvc_ctl *d_ctl;
+  int graph;
 } lvc_batch;
 
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,

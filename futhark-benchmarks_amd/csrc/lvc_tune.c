@@ -66,13 +66,19 @@ static void cache_append(struct futhark_context *ctx, int precision, int nx,
   fclose(f);
 }
 
-/* Built-in choice (measured on MI355X, see the tuning table in the docs). */
+/* Built-in choice: the fastest pair measured on MI355X at 256x256 (128
+ * options, 63 steps): FP32+refinement 8/8 (5.85 ms; 4/4 8.5 ms), FP32 8/16
+ * (4.50 ms), FP64 4/8 (8.00 ms; 16/16 9.76 ms).  Segment 8 keeps a 256-point
+ * system on 32 lanes with the refinement's registers in budget. */
 static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
   (void)nx;
   (void)ny;
   if (precision == LVC_FP64) {
     *sx = 4;
-    *sy = 4;
+    *sy = 8;
+  } else if (precision == LVC_FP32) {
+    *sx = 8;
+    *sy = 16;
   } else {
     *sx = 8;
     *sy = 8;

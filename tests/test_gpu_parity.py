@@ -281,3 +281,18 @@ def test_profiling_report():
         rep = json.loads(ctx.report())
     assert rep["lvc_xsweep"]["count"] == 4 and rep["lvc_ysweep"]["count"] == 4
     assert rep["lvc_xsweep"]["total_us"] > 0
+
+
+
+def test_graph_replay_matches_direct():
+    """The step sequence replayed from a HIP graph gives the same prices as
+    direct launches (bit-identical), for a uniform and an uneven batch."""
+    say("case: graph replay vs direct launches")
+    pr = make_problems(16, seed=9, sizes=(32, 64, 128), nt_range=(4, 12))
+    out = {}
+    for graph in (0, 1):
+        with lvc.LocVolCalib(tuning={"lvc.graph": graph}) as ctx:
+            out[graph] = [ctx.main_batch(**pr, precision=lvc.PRECISION_FP32_REFINE),
+                          ctx.main_f64(8, 64, 64, 9, *CFG.values())]
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
