@@ -53,6 +53,19 @@ def time_calls(ctx, raw, free, steps, warmup):
     return total, lat
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, scaled by a device copy of known size) for this workload,
+    from the committed profile summary; None when there is none."""
+    path = os.path.join(ROOT, "profiles",
+                        "lvc_sweeps_main_128x256x256x64_fp32refine.traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["hbm_bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def pct(v, q):
     import numpy as np
     return float(np.percentile(np.asarray(v) * 1e3, q))
@@ -162,6 +175,8 @@ def main():
                     help="skip the batched and serving measurements")
     ap.add_argument("--serve-rates", default="250,1000,5000",
                     help="Poisson arrival rates (req/s) of the serving runs")
+    ap.add_argument("--only-fp32refine", action="store_true",
+                    help="run only the FP32+refinement calls (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB (read) + 256 MiB "
                          "(write) three times, to scale HBM traffic counters")
@@ -197,6 +212,10 @@ def main():
         total32p, _ = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps, 0)
         rep32 = json.loads(ctx.report())
         ctx.pause_profiling()
+        if args.only_fp32refine:
+            print(json.dumps({"ms_per_step": round(1e3 * total32 / args.steps, 4),
+                              "kernels": rep32}), flush=True)
+            return
         # FP64
         total64, lat64 = time_calls(ctx, ctx.main_f64_raw, ctx.free_f64, args.steps,
                                     args.warmup)
@@ -244,7 +263,8 @@ def main():
         "ms_per_step_with_events": round(1e3 * total32p / args.steps, 4),
         "roofline": {"bound": "hbm", "kernel": dom_name,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dom_name),
                      "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3)},
     }
     if not args.no_forms:

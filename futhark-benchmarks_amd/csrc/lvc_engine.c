@@ -96,8 +96,13 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   int64_t geo = 0, state = 0, nxt = 0, nyt = 0, steps = 0;
   for (int p = 0; p < n; p++) {
     lvc_problem q = probs[p];
-    const int gx = lvc_lanes_for(q.nx, seg_x), gy = lvc_lanes_for(q.ny, seg_y);
-    const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
+    const int gx = lvc_lanes_for(q.nx, seg_x);
+    const int rows = LVC_BLOCK / gx;
+    int gy, cpl, cols;
+    lvc_ytile_shape(q.ny, seg_y, (int)real, &gy, &cpl, &cols);
+    const int lds = lvc_ytile_lds(q.ny, seg_y, (int)real);
+    if (lds > b->ylds)
+      b->ylds = lds;
     q.geo_off = geo;
     q.state_off = state;
     q.k = -1;
@@ -140,14 +145,16 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   for (int oi = 0; oi < n; oi++) {
     const int p = order[oi];
     const lvc_problem *q = &b->h_probs[p];
-    const int gx = lvc_lanes_for(q->nx, seg_x), gy = lvc_lanes_for(q->ny, seg_y);
-    const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
+    const int gx = lvc_lanes_for(q->nx, seg_x);
+    const int rows = LVC_BLOCK / gx;
+    int gy, cpl, cols;
+    lvc_ytile_shape(q->ny, seg_y, (int)real, &gy, &cpl, &cols);
     for (int r0 = 0; r0 < q->ny; r0 += rows) {
-      lvc_tile t = {p, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx};
+      lvc_tile t = {p, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx, 1, 0};
       tiles[tx++] = t;
     }
     for (int c0 = 0; c0 < q->nx; c0 += cols) {
-      lvc_tile t = {p, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy};
+      lvc_tile t = {p, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy, cpl, 0};
       tiles[ty++] = t;
     }
     /* this problem runs steps 0 .. nt-2: the prefixes reach past its tiles */
@@ -205,7 +212,7 @@ static int run_graph(struct futhark_context *ctx, lvc_batch *b,
                           b->x_active[s], b->d_geo, b->d_R, b->d_Y);
     if (!e)
       e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
-                            b->y_active[s], b->d_geo, b->d_Y, b->d_R);
+                            b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds);
   }
   const int e2 = (int)hipStreamEndCapture(ctx->stream, &graph);
   rc = lvc_check_hip(ctx, e, "sweep kernel (capture)") ||
@@ -243,7 +250,7 @@ int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
       return 1;
     tok = lvc_prof_begin(ctx, "lvc_ysweep");
     e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
-                          b->y_active[s], b->d_geo, b->d_Y, b->d_R);
+                          b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "y sweep kernel"))
       return 1;

@@ -55,6 +55,8 @@ typedef struct lvc_tile {
   int32_t first;
   int32_t count;
   int32_t lanes;
+  int32_t cpl; /* y sweep: columns per lane (they share the factorisation) */
+  int32_t pad_;
 } lvc_tile;
 
 #define LVC_P4(n) (((int64_t)(n) + 3) & ~(int64_t)3)
@@ -74,7 +76,8 @@ int lvc_launch_xsweep(void *stream, int precision, int seg,
                       int ntiles, const double *geo, const void *R, void *Y);
 int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
-                      int ntiles, const double *geo, const void *Y, void *R);
+                      int ntiles, const double *geo, const void *Y, void *R,
+                      int lds_bytes);
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
                       int nprob, const void *R, void *out, int out_f64);
 
@@ -85,6 +88,32 @@ static inline int lvc_lanes_for(int n, int seg) {
   while (G < P)
     G <<= 1;
   return G;
+}
+
+/* Shape of a y-sweep tile for columns of length ny: G lanes per column,
+ * cpl columns per lane, cols = cpl * (LVC_BLOCK / G) columns per workgroup
+ * (aiming at a 128-byte row segment: 32 floats or 16 doubles), with the LDS
+ * panel of G*seg rows x (cols + 1) kept within 64 KiB.  cols is a power of
+ * two dividing LVC_BLOCK. */
+static inline void lvc_ytile_shape(int ny, int seg, int real, int *G, int *cpl,
+                                   int *cols) {
+  const int g = lvc_lanes_for(ny, seg);
+  const int lc = LVC_BLOCK / g;
+  int c = 1; /* a power of two, so that cols divides LVC_BLOCK */
+  while (2 * c * lc <= (real == 4 ? 32 : 16))
+    c *= 2;
+  while (c > 1 && (long long)g * seg * (c * lc + 1) * real > 65536)
+    c /= 2;
+  *G = g;
+  *cpl = c;
+  *cols = c * lc;
+}
+
+/* LDS bytes of a y-sweep tile of that shape. */
+static inline int lvc_ytile_lds(int ny, int seg, int real) {
+  int G, cpl, cols;
+  lvc_ytile_shape(ny, seg, real, &G, &cpl, &cols);
+  return G * seg * (cols + 1) * real;
 }
 
 #ifdef __cplusplus

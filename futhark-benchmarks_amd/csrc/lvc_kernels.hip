@@ -87,8 +87,24 @@ __device__ __forceinline__ double dpp(double old, double v) {
 // Lanes of one system: `seg` = this lane's index in its group of `G` lanes.
 struct Lanes {
   int seg, G;
-  bool row;  // G <= 16: the group lies inside one DPP row
+  bool row;   // G <= 16: the group lies inside one DPP row
+  int lane4;  // 4 * (lane index in the wavefront): ds_bpermute address base
 };
+
+__device__ __forceinline__ Lanes make_lanes(int seg, int G) {
+  return Lanes{seg, G, G <= 16, (int)__lane_id() << 2};
+}
+
+// ds_bpermute: the value of lane (addr >> 2) mod 64.
+__device__ __forceinline__ float bperm(int addr, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double bperm(int addr, double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)b);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // Value of lane seg-st of the group (ident below the group's first lane).
 template <typename T>
@@ -102,7 +118,7 @@ __device__ __forceinline__ T from_below(const Lanes &L, T v, int st, T ident) {
       default: r = dpp<0x118>(ident, v); break;
     }
   } else {
-    r = __shfl_up(v, st, L.G);
+    r = bperm(L.lane4 - 4 * st, v);  // outside the group: discarded below
   }
   return L.seg >= st ? r : ident;
 }
@@ -119,7 +135,7 @@ __device__ __forceinline__ T from_above(const Lanes &L, T v, int st, T ident) {
       default: r = dpp<0x108>(ident, v); break;
     }
   } else {
-    r = __shfl_down(v, st, L.G);
+    r = bperm(L.lane4 + 4 * st, v);  // outside the group: discarded below
   }
   return L.seg + st < L.G ? r : ident;
 }
@@ -465,7 +481,7 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
     }
   }
   SegSys<T, M> sys;
-  sys.prepare(a, c, s, Lanes{seg, G, G <= 16});
+  sys.prepare(a, c, s, make_lanes(seg, G));
   T x[M];
   sys.solve(U, x);
   if (REF) {
@@ -489,13 +505,14 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// y sweep: one workgroup = `count` columns of one problem, `lanes` lanes per
-// column; the column panel is staged through LDS (coalesced row loads and
-// stores), stored segment-interleaved so that lanes of one column read
-// different banks.
+// y sweep: one workgroup = `count` columns of one problem.  The y matrix is
+// the same for every column of a problem (VarY = nu^2, :59), so a lane
+// factors its segment once and solves `cpl` columns with it; G lanes per
+// column, cpl * 256/G columns per workgroup.  The column panel is staged
+// through LDS (coalesced row loads and stores), stored segment-interleaved
+// (row r = q*M + k at panel row k*G + q) so the lanes of a column read
+// consecutive panel rows.
 // ---------------------------------------------------------------------------
-template <typename T, int M>
-constexpr int kYsweepLds = M * (LVC_BLOCK + 64);  // elements of T
 
 // Every thread of the workgroup must call this (it holds barriers).
 template <typename T, int M, bool REF>
@@ -506,20 +523,23 @@ __device__ __forceinline__ void ysweep_tile(const lvc_tile tile, lvc_problem *pr
   lvc_problem &pr = probs[tile.problem];
   const int kt = pr.k;
   if (kt < 0) return;
-  const int G = tile.lanes;
-  const int cols = LVC_BLOCK / G;  // columns per workgroup
+  const int G = tile.lanes, cpl = tile.cpl;
+  const int lc = LVC_BLOCK / G;    // lane-columns per workgroup
+  const int cols = cpl * lc;       // columns per workgroup
   const int CP = cols + 1;         // LDS pitch
   const int nx = pr.nx, ny = pr.ny;
   const int ncol = tile.count, col0 = tile.first;
   const int rows_pad = G * M;
-  auto lidx = [&](int r, int cc) { return ((r % M) * G + r / M) * CP + cc; };
+  auto lidx = [&](int r, int c) { return ((r % M) * G + r / M) * CP + c; };
 
-  const T *src = Y + pr.state_off;
-  for (int e = threadIdx.x; e < rows_pad * cols; e += LVC_BLOCK) {
-    const int r = e / cols, cc = e % cols;
+  // cols divides LVC_BLOCK: each thread keeps one column and strides rows
+  const int pc = threadIdx.x & (cols - 1), pr0 = threadIdx.x / cols;
+  const int rstride = LVC_BLOCK / cols;
+  const T *src = Y + pr.state_off + col0 + pc;
+  for (int r = pr0; r < rows_pad; r += rstride) {
     T val = T(0);
-    if (r < ny && cc < ncol) val = src[(int64_t)r * nx + col0 + cc];
-    lds[lidx(r, cc)] = val;
+    if (r < ny && pc < ncol) val = src[(int64_t)r * nx];
+    lds[lidx(r, pc)] = val;
   }
   __syncthreads();
 
@@ -527,35 +547,31 @@ __device__ __forceinline__ void ysweep_tile(const lvc_tile tile, lvc_problem *pr
   const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
   const T dt = (T)(1.0 / (tnext - tnow));
   const int q = threadIdx.x & (G - 1);
-  const int cc = threadIdx.x / G;
+  const int lcol = threadIdx.x / G;
   const int j0 = q * M;
-  T a[M], c[M], s[M], d[M];
+  T a[M], c[M], s[M];
   const bool vec = ((nx | ny) & 3) == 0;
   load_seg<T, M>(tp.ay, j0, ny, vec, a);
   load_seg<T, M>(tp.cy, j0, ny, vec, c);
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
-    s[k] = j0 + k < ny ? dt : T(1);
-    d[k] = lds[lidx(j0 + k, cc)];
-  }
+  for (int k = 0; k < M; ++k) s[k] = j0 + k < ny ? dt : T(1);
   SegSys<T, M> sys;
-  sys.prepare(a, c, s, Lanes{q, G, G <= 16});
-  T x[M];
-  sys.solve(d, x);
-  if (REF) {
+  sys.prepare(a, c, s, make_lanes(q, G));
+  for (int ci = 0; ci < cpl; ++ci) {
+    const int col = ci * lc + lcol;  // lanes of a column interleave across cpl
+    T d[M], x[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) d[k] = lds[lidx(j0 + k, cc)];
-    refine(sys, d, a, c, s, x);
+    for (int k = 0; k < M; ++k) d[k] = lds[lidx(j0 + k, col)];
+    sys.solve(d, x);
+    if (REF) refine(sys, d, a, c, s, x);
+    // each lane overwrites only its own panel entries, read just above
+#pragma unroll
+    for (int k = 0; k < M; ++k) lds[lidx(j0 + k, col)] = x[k];
   }
-  __syncthreads();  // every lane has read its panel values
-#pragma unroll
-  for (int k = 0; k < M; ++k) lds[lidx(j0 + k, cc)] = x[k];
   __syncthreads();
-  T *dst = R + pr.state_off;
-  for (int e = threadIdx.x; e < ny * cols; e += LVC_BLOCK) {
-    const int r = e / cols, c2 = e % cols;
-    if (c2 < ncol) dst[(int64_t)r * nx + col0 + c2] = lds[lidx(r, c2)];
-  }
+  T *dst = R + pr.state_off + col0 + pc;
+  if (pc < ncol)
+    for (int r = pr0; r < ny; r += rstride) dst[(int64_t)r * nx] = lds[lidx(r, pc)];
   // The last of the problem's y workgroups to finish advances its time index
   // (every workgroup of the problem has read `k` before it counts itself).
   if (threadIdx.x == 0) {
@@ -571,8 +587,9 @@ template <typename T, int M, bool REF>
 __global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
     lvc_problem *probs, const lvc_tile *__restrict__ tiles,
     const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R) {
-  __shared__ T smem[kYsweepLds<T, M>];
-  ysweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, Y, R, smem);
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  ysweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, Y, R,
+                         reinterpret_cast<T *>(dyn_lds));
 }
 
 // value (:236): the price at (yindex, xindex).
@@ -592,36 +609,38 @@ __global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
 template <typename T, int M, bool REF>
 int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
                   const lvc_tile *tiles, int ntiles, const double *geo,
-                  const void *in, void *out) {
+                  const void *in, void *out, int lds_bytes) {
   if (ntiles <= 0) return 0;
-  if (xs)
+  if (xs) {
     lvc_xsweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, 0, st>>>(
         probs, tiles, geo, (const T *)in, (T *)out);
-  else
-    lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, 0, st>>>(
+  } else {
+    if (lds_bytes <= 0 || lds_bytes > 65536) return (int)hipErrorInvalidValue;
+    lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, lds_bytes, st>>>(
         const_cast<lvc_problem *>(probs), tiles, geo, (const T *)in, (T *)out);
+  }
   return (int)hipGetLastError();
 }
 
 template <typename T, bool REF>
 int dispatch_seg(hipStream_t st, bool xs, int seg, const lvc_problem *probs,
                  const lvc_tile *tiles, int ntiles, const double *geo,
-                 const void *in, void *out) {
+                 const void *in, void *out, int lds) {
   switch (seg) {
-    case 4: return launch_sweeps<T, 4, REF>(st, xs, probs, tiles, ntiles, geo, in, out);
-    case 8: return launch_sweeps<T, 8, REF>(st, xs, probs, tiles, ntiles, geo, in, out);
-    case 16: return launch_sweeps<T, 16, REF>(st, xs, probs, tiles, ntiles, geo, in, out);
+    case 4: return launch_sweeps<T, 4, REF>(st, xs, probs, tiles, ntiles, geo, in, out, lds);
+    case 8: return launch_sweeps<T, 8, REF>(st, xs, probs, tiles, ntiles, geo, in, out, lds);
+    case 16: return launch_sweeps<T, 16, REF>(st, xs, probs, tiles, ntiles, geo, in, out, lds);
     default: return (int)hipErrorInvalidValue;
   }
 }
 
 int dispatch(hipStream_t st, bool xs, int precision, int seg,
              const lvc_problem *probs, const lvc_tile *tiles, int ntiles,
-             const double *geo, const void *in, void *out) {
+             const double *geo, const void *in, void *out, int lds) {
   switch (precision) {
-    case LVC_FP32: return dispatch_seg<float, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out);
-    case LVC_FP32_REFINE: return dispatch_seg<float, true>(st, xs, seg, probs, tiles, ntiles, geo, in, out);
-    case LVC_FP64: return dispatch_seg<double, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out);
+    case LVC_FP32: return dispatch_seg<float, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out, lds);
+    case LVC_FP32_REFINE: return dispatch_seg<float, true>(st, xs, seg, probs, tiles, ntiles, geo, in, out, lds);
+    case LVC_FP64: return dispatch_seg<double, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out, lds);
     default: return (int)hipErrorInvalidValue;
   }
 }
@@ -648,14 +667,15 @@ int lvc_launch_xsweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *R, void *Y) {
   return dispatch((hipStream_t)stream, true, precision, seg, probs, tiles,
-                  ntiles, geo, R, Y);
+                  ntiles, geo, R, Y, 0);
 }
 
 int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
-                      int ntiles, const double *geo, const void *Y, void *R) {
+                      int ntiles, const double *geo, const void *Y, void *R,
+                      int lds_bytes) {
   return dispatch((hipStream_t)stream, false, precision, seg, probs, tiles,
-                  ntiles, geo, Y, R);
+                  ntiles, geo, Y, R, lds_bytes);
 }
 
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,

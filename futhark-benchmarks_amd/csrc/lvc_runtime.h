@@ -120,6 +120,7 @@ typedef struct lvc_batch {
   /* tiles of the problems still running at step s: a prefix of each list */
   int *x_active, *y_active; /* [steps] */
   int graph;
+  int ylds; /* LDS bytes of the y-sweep launches */
 } lvc_batch;
 
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,

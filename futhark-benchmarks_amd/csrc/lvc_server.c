@@ -55,7 +55,7 @@ struct lvc_server {
   double *h_out;
   /* tiles of the resident set */
   lvc_tile *h_tiles, *d_tiles;
-  int nxt, nyt, tiles_cap, tiles_dirty;
+  int nxt, nyt, tiles_cap, tiles_dirty, ylds;
   /* queue and results */
   queued_req *queue;
   int64_t qhead, qtail, qcap;
@@ -227,18 +227,24 @@ static int rebuild_tiles(struct lvc_server *s) {
     }
   }
   int tx = 0, ty = nx_t;
+  s->ylds = 0;
   for (int i = 0; i < s->max_slots; i++) {
     if (s->slot_id[i] < 0)
       continue;
     const lvc_problem *q = &s->h_slots[i];
-    const int gx = lvc_lanes_for(q->nx, s->seg_x), gy = lvc_lanes_for(q->ny, s->seg_y);
-    const int rows = LVC_BLOCK / gx, cols = LVC_BLOCK / gy;
+    const int gx = lvc_lanes_for(q->nx, s->seg_x);
+    const int rows = LVC_BLOCK / gx;
+    int gy, cpl, cols;
+    lvc_ytile_shape(q->ny, s->seg_y, (int)s->real, &gy, &cpl, &cols);
+    const int lds = lvc_ytile_lds(q->ny, s->seg_y, (int)s->real);
+    if (lds > s->ylds)
+      s->ylds = lds;
     for (int r0 = 0; r0 < q->ny; r0 += rows) {
-      lvc_tile t = {i, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx};
+      lvc_tile t = {i, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx, 1, 0};
       s->h_tiles[tx++] = t;
     }
     for (int c0 = 0; c0 < q->nx; c0 += cols) {
-      lvc_tile t = {i, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy};
+      lvc_tile t = {i, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy, cpl, 0};
       s->h_tiles[ty++] = t;
     }
   }
@@ -272,8 +278,11 @@ static int admit(struct lvc_server *s) {
     q.k = -1;
     q.state_off = (int64_t)slot * s->slot_elems;
     q.geo_off = (int64_t)slot * s->slot_geo;
-    q.ytiles = (q.nx + LVC_BLOCK / lvc_lanes_for(q.ny, s->seg_y) - 1) /
-               (LVC_BLOCK / lvc_lanes_for(q.ny, s->seg_y));
+    {
+      int gy, cpl, cols;
+      lvc_ytile_shape(q.ny, s->seg_y, (int)s->real, &gy, &cpl, &cols);
+      q.ytiles = (q.nx + cols - 1) / cols;
+    }
     s->h_slots[slot] = q;
     s->slot_id[slot] = qr->id;
     s->steps_left[slot] = q.nt - 1;
@@ -311,7 +320,8 @@ int32_t lvc_server_step(struct lvc_server *s) {
     if (!rc) {
       tok = lvc_prof_begin(ctx, "lvc_ysweep");
       e = lvc_launch_ysweep(st, s->precision, s->seg_y, s->d_slots,
-                            s->d_tiles + s->nxt, s->nyt, s->d_geo, s->d_Y, s->d_R);
+                            s->d_tiles + s->nxt, s->nyt, s->d_geo, s->d_Y, s->d_R,
+                            s->ylds);
       lvc_prof_end(ctx, tok);
       rc = lvc_check_hip(ctx, e, "y sweep kernel");
     }

@@ -67,21 +67,22 @@ static void cache_append(struct futhark_context *ctx, int precision, int nx,
 }
 
 /* Built-in choice: the fastest pair measured on MI355X at 256x256 (128
- * options, 63 steps): FP32+refinement 8/8 (5.85 ms; 4/4 8.5 ms), FP32 8/16
- * (4.50 ms), FP64 4/8 (8.00 ms; 16/16 9.76 ms).  Segment 8 keeps a 256-point
- * system on 32 lanes with the refinement's registers in budget. */
+ * options, 63 steps): FP32+refinement 8/4 (4.04 ms; 8/8 4.63, 4/4 4.61),
+ * FP32 8/8 (3.09 ms), FP64 4/4 (5.76 ms; 8/8 6.44).  x segments of 8 keep a
+ * 256-point row on 32 lanes; y segments of 4 put a column on 64 lanes, each
+ * solving 8 columns with one factorisation. */
 static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
   (void)nx;
   (void)ny;
   if (precision == LVC_FP64) {
     *sx = 4;
-    *sy = 8;
+    *sy = 4;
   } else if (precision == LVC_FP32) {
     *sx = 8;
-    *sy = 16;
+    *sy = 8;
   } else {
     *sx = 8;
-    *sy = 8;
+    *sy = 4;
   }
 }
 
