@@ -41,12 +41,16 @@ int lvc_validate_sizes(struct futhark_context *ctx, int64_t nx, int64_t ny,
   return 0;
 }
 
-/* Smallest supported segment length >= want whose lane count fits a
- * wavefront (<= 64 lanes per system). */
-static int fit_seg(int n, int want) {
-  static const int segs[] = {4, 8, 16};
-  for (int i = 0; i < 3; i++)
-    if (segs[i] >= want && lvc_lanes_for(n, segs[i]) <= 64)
+/* Smallest supported segment length >= want that fits a system of length n:
+ * x sweep {4, 8, 16} (<= 64 lanes per system); y sweep also the column
+ * segments {32, 64} (<= LVC_COL_BLOCK/64 segments per column), falling back
+ * to 16 when no column segment fits. */
+static int fit_seg(int n, int want, int ysweep) {
+  static const int segs[] = {4, 8, 16, 32, 64};
+  const int ns = ysweep ? 5 : 3;
+  for (int i = 0; i < ns; i++)
+    if (segs[i] >= want && (ysweep ? lvc_yseg_fits(n, segs[i])
+                                   : lvc_lanes_for(n, segs[i]) <= 64))
       return segs[i];
   return 16;
 }
@@ -57,8 +61,8 @@ void lvc_choose_segs(struct futhark_context *ctx, int precision, int nx, int ny,
   int tx = 0, ty = 0;
   if (!sx || !sy)
     lvc_tune_pick(ctx, precision, nx, ny, &tx, &ty);
-  *seg_x = fit_seg(nx, sx ? sx : tx);
-  *seg_y = fit_seg(ny, sy ? sy : ty);
+  *seg_x = fit_seg(nx, sx ? sx : tx, 0);
+  *seg_y = fit_seg(ny, sy ? sy : ty, 1);
 }
 
 static int64_t align64(int64_t v) { return (v + 63) & ~(int64_t)63; }
@@ -128,7 +132,8 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   int *order = malloc((size_t)n * sizeof(int));
   b->x_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
   b->y_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
-  if (!tiles || !order || !b->x_active || !b->y_active) {
+  b->f_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
+  if (!tiles || !order || !b->x_active || !b->y_active || !b->f_active) {
     free(tiles);
     free(order);
     lvc_set_error(ctx, "host allocation failed");
@@ -141,7 +146,7 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   nt_sort_desc(order, n, b->h_probs);
   int64_t tx = 0, ty = nxt;
   for (int64_t s = 0; s < steps; s++)
-    b->x_active[s] = b->y_active[s] = 0;
+    b->x_active[s] = b->y_active[s] = b->f_active[s] = 0;
   for (int oi = 0; oi < n; oi++) {
     const int p = order[oi];
     const lvc_problem *q = &b->h_probs[p];
@@ -161,22 +166,28 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     for (int64_t s = 0; s < q->nt - 1; s++) {
       b->x_active[s] = (int)tx;
       b->y_active[s] = (int)(ty - nxt);
+      b->f_active[s] = oi + 1;
     }
   }
-  free(order);
   b->probs_bytes = (size_t)n * sizeof(lvc_problem);
   b->geo_bytes = (size_t)geo * sizeof(double);
   b->state_bytes = (size_t)state * real;
   b->tiles_bytes = (size_t)(nxt + nyt) * sizeof(lvc_tile);
+  b->yfac_bytes = lvc_seg_is_col(seg_y) ? (size_t)n * LVC_YFAC_LEN * real : 0;
   int rc = lvc_dev_alloc(ctx, b->probs_bytes, (void **)&b->d_probs) ||
            lvc_dev_alloc(ctx, b->geo_bytes, (void **)&b->d_geo) ||
            lvc_dev_alloc(ctx, b->state_bytes, &b->d_R) ||
            lvc_dev_alloc(ctx, b->state_bytes, &b->d_Y) ||
-           lvc_dev_alloc(ctx, b->tiles_bytes, (void **)&b->d_tiles);
+           lvc_dev_alloc(ctx, b->tiles_bytes, (void **)&b->d_tiles) ||
+           (b->yfac_bytes &&
+            (lvc_dev_alloc(ctx, b->yfac_bytes, &b->d_yfac) ||
+             lvc_dev_alloc(ctx, (size_t)n * sizeof(int32_t), (void **)&b->d_fprobs)));
   if (!rc)
     rc = lvc_upload(ctx, b->d_probs, b->h_probs, b->probs_bytes) ||
-         lvc_upload(ctx, b->d_tiles, tiles, b->tiles_bytes);
+         lvc_upload(ctx, b->d_tiles, tiles, b->tiles_bytes) ||
+         (b->d_fprobs && lvc_upload(ctx, b->d_fprobs, order, (size_t)n * sizeof(int32_t)));
   free(tiles);
+  free(order);
   b->graph = ctx->cfg && ctx->cfg->graph;
   return rc;
 }
@@ -187,9 +198,14 @@ void lvc_batch_release(struct futhark_context *ctx, lvc_batch *b) {
   lvc_dev_free(ctx, b->d_R, b->state_bytes);
   lvc_dev_free(ctx, b->d_Y, b->state_bytes);
   lvc_dev_free(ctx, b->d_tiles, b->tiles_bytes);
+  if (b->d_yfac)
+    lvc_dev_free(ctx, b->d_yfac, b->yfac_bytes);
+  if (b->d_fprobs)
+    lvc_dev_free(ctx, b->d_fprobs, (size_t)b->n * sizeof(int32_t));
   free(b->h_probs);
   free(b->x_active);
   free(b->y_active);
+  free(b->f_active);
   memset(b, 0, sizeof(*b));
 }
 
@@ -209,10 +225,13 @@ static int run_graph(struct futhark_context *ctx, lvc_batch *b,
   int e = 0;
   for (int64_t s = 0; !e && s < b->steps; s++) {
     e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
-                          b->x_active[s], b->d_geo, b->d_R, b->d_Y);
+                          b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
+                          lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
+                          b->f_active[s]);
     if (!e)
       e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
-                            b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds);
+                            b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds,
+                            b->d_yfac);
   }
   const int e2 = (int)hipStreamEndCapture(ctx->stream, &graph);
   rc = lvc_check_hip(ctx, e, "sweep kernel (capture)") ||
@@ -244,13 +263,16 @@ int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
   for (int64_t s = 0; s < b->steps; s++) {
     tok = lvc_prof_begin(ctx, "lvc_xsweep");
     e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
-                          b->x_active[s], b->d_geo, b->d_R, b->d_Y);
+                          b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
+                          lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
+                          b->f_active[s]);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "x sweep kernel"))
       return 1;
     tok = lvc_prof_begin(ctx, "lvc_ysweep");
     e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
-                          b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds);
+                          b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds,
+                          b->d_yfac);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "y sweep kernel"))
       return 1;

@@ -62,22 +62,35 @@ typedef struct lvc_tile {
 #define LVC_P4(n) (((int64_t)(n) + 3) & ~(int64_t)3)
 #define LVC_GEO_F64(nx, ny) ((3 * (int64_t)(nx) + 3 * (int64_t)(ny) + 1) & ~(int64_t)1)
 #define LVC_GEO_LEN(nx, ny) (LVC_GEO_F64(nx, ny) + LVC_P4(nx) + LVC_P4(ny))
-#define LVC_BLOCK 256 /* threads per workgroup of the sweeps */
+#define LVC_BLOCK 256 /* threads per workgroup of the lane-segment sweeps */
+#define LVC_COL_BLOCK 512 /* threads per workgroup of the column y sweep */
+#define LVC_COL_ROWS 512  /* rows of a column system: 8 segments of <= 64 */
+/* real elements of one problem's y factor table (column y sweep) */
+#define LVC_YFAC_LEN (6 * LVC_COL_ROWS + 6 * (LVC_COL_BLOCK / 64))
 #define LVC_MAX_N 1024 /* largest numX / numY accepted (64 lanes x 16) */
 
-/* Supported segment lengths (elements of one system per lane). */
-int lvc_seg_supported(int seg);
+/* Supported segment lengths (elements of one system per lane): 4, 8, 16 for
+ * both sweeps (the lanes of a system are lanes of one wavefront, reduced
+ * system by PCR); 32 and 64 for the y sweep only (one lane per column
+ * segment, the segments of a column in different wavefronts, reduced system
+ * by Thomas). */
+int lvc_seg_supported(int seg, int ysweep);
 
 /* Launchers: return 0 or a hipError_t value.  `stream` is a hipStream_t. */
 int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
                      double *geo, void *R);
+/* x sweep; with yseg = a column segment length (32, 64) it also writes the
+ * y factor table for the step of each of the nfac problems fprobs[] into
+ * yfac (real elements, LVC_YFAC_LEN per problem, indexed by problem). */
 int lvc_launch_xsweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
-                      int ntiles, const double *geo, const void *R, void *Y);
+                      int ntiles, const double *geo, const void *R, void *Y,
+                      void *yfac, int yseg, const int32_t *fprobs, int nfac);
+/* y sweep; column segments (seg >= 32) read the factor tables in yfac. */
 int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *Y, void *R,
-                      int lds_bytes);
+                      int lds_bytes, const void *yfac);
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
                       int nprob, const void *R, void *out, int out_f64);
 
@@ -90,13 +103,30 @@ static inline int lvc_lanes_for(int n, int seg) {
   return G;
 }
 
-/* Shape of a y-sweep tile for columns of length ny: G lanes per column,
+/* Column y sweep (seg >= 32): segments per column and whether ny fits. */
+static inline int lvc_seg_is_col(int seg) { return seg >= 32; }
+static inline int lvc_yseg_fits(int ny, int seg) {
+  if (lvc_seg_is_col(seg))
+    return (ny + seg - 1) / seg <= LVC_COL_BLOCK / 64;
+  return lvc_lanes_for(ny, seg) <= 64;
+}
+
+/* Shape of a y-sweep tile for columns of length ny.  Column sweep (seg >= 32):
+ * G = ceil(ny/seg) segments (wavefronts) per column, cpl = column groups of
+ * 64 columns per workgroup, cols = 64 * cpl.  Lane-segment sweep: G lanes per column,
  * cpl columns per lane, cols = cpl * (LVC_BLOCK / G) columns per workgroup
  * (aiming at a 128-byte row segment: 32 floats or 16 doubles), with the LDS
  * panel of G*seg rows x (cols + 1) kept within 64 KiB.  cols is a power of
  * two dividing LVC_BLOCK. */
 static inline void lvc_ytile_shape(int ny, int seg, int real, int *G, int *cpl,
                                    int *cols) {
+  if (lvc_seg_is_col(seg)) {
+    const int P = (ny + seg - 1) / seg;
+    *G = P;
+    *cpl = (LVC_COL_BLOCK / 64) / P;
+    *cols = 64 * *cpl;
+    return;
+  }
   const int g = lvc_lanes_for(ny, seg);
   const int lc = LVC_BLOCK / g;
   int c = 1; /* a power of two, so that cols divides LVC_BLOCK */
@@ -109,8 +139,11 @@ static inline void lvc_ytile_shape(int ny, int seg, int real, int *G, int *cpl,
   *cols = c * lc;
 }
 
-/* LDS bytes of a y-sweep tile of that shape. */
+/* Dynamic LDS bytes of a y-sweep tile of that shape (the column sweep's LDS
+ * is static). */
 static inline int lvc_ytile_lds(int ny, int seg, int real) {
+  if (lvc_seg_is_col(seg))
+    return 0;
   int G, cpl, cols;
   lvc_ytile_shape(ny, seg, real, &G, &cpl, &cols);
   return G * seg * (cols + 1) * real;

@@ -317,7 +317,7 @@ struct SegSys {
     for (int k = 0; k < M; ++k) c[k] = cc[k];
     aL = a[M - 1];
     cL = c[M - 1];
-    // Local factorisation of rows 0..M-2: g = pivot + |c|, every update a sum
+    // Local factorisation of rows 0..M-2: g = pivot - |c|, every update a sum
     // of positive terms for an M-matrix.
     T g = s[0] - a[0];
     rb[0] = rcp(g - c[0]);
@@ -496,12 +496,38 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
   store_seg<T, M>(Y + pr.state_off + (int64_t)j * nx, i0, nx, vec, out);
 }
 
+constexpr int kFacScratch = 6 * LVC_COL_ROWS;  // elements of the factor scratch
+template <typename T>
+__device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
+                            T *__restrict__ f, T *scr);
+
+// yseg > 0: the y sweep runs on column segments of yseg rows, and the first
+// nfac workgroups of the grid each write one problem's y factor for this step
+// (problem fprobs[blockIdx.x], wave 0 at raised priority: a serial chain that
+// should finish long before the row tiles do); the row tiles follow.
 template <typename T, int M, bool REF>
 __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
     const lvc_problem *__restrict__ probs, const lvc_tile *__restrict__ tiles,
-    const double *__restrict__ geo, const T *__restrict__ R, T *__restrict__ Y) {
+    const double *__restrict__ geo, const T *__restrict__ R, T *__restrict__ Y,
+    T *__restrict__ yfac, int yseg, const int32_t *__restrict__ fprobs, int nfac) {
   __shared__ T smem[kXsweepLds<T, M>];
-  xsweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, R, Y, smem);
+  __shared__ T fscr[kFacScratch];  // scratch of the factor workgroups
+  if ((int)blockIdx.x < nfac) {
+    if (yseg > 0 && threadIdx.x < 64) {
+      const int p = fprobs[blockIdx.x];
+      const lvc_problem &pr = probs[p];
+      const int kt = pr.k;
+      if (kt >= 0) {
+        __builtin_amdgcn_s_setprio(3);
+        const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
+        ycol_factor<T>(pr, geo + pr.geo_off, (T)(1.0 / (tnext - tnow)), yseg,
+                       yfac + (int64_t)p * LVC_YFAC_LEN, fscr);
+        __builtin_amdgcn_s_setprio(0);
+      }
+    }
+    return;
+  }
+  xsweep_tile<T, M, REF>(tiles[blockIdx.x - nfac], probs, geo, R, Y, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -592,6 +618,336 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
                          reinterpret_cast<T *>(dyn_lds));
 }
 
+// ---------------------------------------------------------------------------
+// Column y sweep (segment lengths 32 and 64): one lane per column segment,
+// Thomas elimination down each segment.  A workgroup is LVC_COL_BLOCK / 64
+// waves; wave w holds segment p = w % P of the 64 columns of column group
+// w / P (P = ceil(ny / M) segments per column), lane = column.  Every lane of
+// a wave works on the same rows, so the factorisation, which is the same for
+// every column (VarY = nu^2, :59), is computed once per segment into LDS and
+// read at wave-uniform addresses; the column data itself stays in registers
+// (coalesced row loads and stores, no transpose).  The P segments of a column
+// are coupled through their last unknowns: a tridiagonal reduced system of P
+// rows, which every lane solves for its own column by Thomas after one LDS
+// exchange.  With P = 1 this is plain thread-per-column Thomas.
+//
+// Segment algebra (rows k = 0..M-1 of segment p, global row p*M + k): rows
+// 0..M-2 give x_k = y_k + v_k X_{p-1} + w_k X_p, where y solves them with the
+// couplings to X_{p-1} (the previous segment's last unknown) and X_p (this
+// segment's last) set to zero, and v, w are the responses to a unit value of
+// each; row M-1 then couples X_{p-1}, X_p and X_{p+1}.  Long segments of a
+// weakly shifted system have v, w close to 1, so the reduced diagonal is not
+// formed from 1 - w or 1 - v: the rows of the difference form sum to s, which
+// gives 1 = v + w + z with z the response to the shift s itself (a sum of
+// positive terms), and the reduced system is factored in the same g-form from
+// its row sums  s - aL*z_{M-2} - cL*z'_0  (z' of the next segment).
+// ---------------------------------------------------------------------------
+constexpr int kColWaves = LVC_COL_BLOCK / 64;  // segments per column, at most
+static_assert((kColWaves & (kColWaves - 1)) == 0, "wave count must be a power of two");
+
+// The factor table of one problem's y system (elements of T, at problem *
+// LVC_YFAC_LEN): per row m, rb, crb, v, w, z (LVC_COL_ROWS rows each), then
+// per segment aL, cL, sL (taps of its last row) and the reduced system's
+// Thomas factor rm, rrb, rC (kColWaves entries each).
+enum { kFm, kFrb, kFcrb, kFv, kFw, kFz };
+enum { kSaL, kScL, kSsL, kSrm, kSrrb, kSrC };
+static_assert(LVC_YFAC_LEN == 6 * LVC_COL_ROWS + 6 * kColWaves, "factor table layout");
+static_assert(LVC_COL_ROWS >= kColWaves * 64, "largest column segment is 64 rows");
+
+template <typename T>
+__device__ __forceinline__ T *fac_row(T *f, int which) { return f + which * LVC_COL_ROWS; }
+template <typename T>
+__device__ __forceinline__ T *fac_seg(T *f, int which) {
+  return f + 6 * LVC_COL_ROWS + which * kColWaves;
+}
+
+// Factorisation of one problem's y system for segments of M rows (the time
+// step's dt), written to its factor table.  Called by all 64 lanes of one
+// wavefront; lane q < P factors segment q (Thomas in g-form, g = pivot - |c|,
+// every update a sum of positive terms; responses v, w, z), then every lane
+// forms the reduced system and lane 0 writes its factor.  `scr` is this
+// wavefront's LDS scratch of kFacScratch elements: the taps, staged once with
+// coalesced loads so that the serial loops never wait on global memory, and
+// the forward values the backward pass reads back (each lane its own rows).
+
+template <typename T>
+__device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
+                            T *__restrict__ f, T *scr) {
+  const int ny = pr.ny, P = (ny + M - 1) / M;
+  const int q = __lane_id();
+  T *srb = scr, *scrb = scr + LVC_COL_ROWS, *sv = scr + 2 * LVC_COL_ROWS;
+  T *sz = scr + 3 * LVC_COL_ROWS, *sa = scr + 4 * LVC_COL_ROWS;
+  T *sc = scr + 5 * LVC_COL_ROWS;
+  {
+    const Taps<T> tp(g, pr.nx, ny);
+    for (int r = q; r < P * M; r += 64) {
+      sa[r] = r < ny ? tp.ay[r] : T(0);
+      sc[r] = r < ny ? tp.cy[r] : T(0);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  auto A = [&](int r) { return sa[r]; };
+  auto C = [&](int r) { return sc[r]; };
+  auto S = [&](int r) { return r < ny ? dt : T(1); };
+  T *fm = fac_row(f, kFm), *frb = fac_row(f, kFrb), *fcrb = fac_row(f, kFcrb);
+  T *fv = fac_row(f, kFv), *fw = fac_row(f, kFw), *fz = fac_row(f, kFz);
+  T vL = T(0), zL = T(0), w0 = T(0), z0 = T(0), aL = T(0), cL = T(0), sL = T(1);
+  if (q < P) {
+    const int r0 = q * M;
+    T a = A(r0), c = C(r0);
+    T gg = S(r0) - a;
+    T rb = rcp(gg - c);
+    T v = -a, z = S(r0);
+    fm[r0] = T(0);
+    frb[r0] = srb[r0] = rb;
+    fcrb[r0] = scrb[r0] = c * rb;
+    sv[r0] = v;
+    sz[r0] = z;
+#pragma unroll 8
+    for (int k = 1; k < M - 1; ++k) {
+      const int r = r0 + k;
+      a = A(r);
+      c = C(r);
+      const T m = a * rb;
+      gg = S(r) - m * gg;
+      rb = rcp(gg - c);
+      v = -m * v;
+      z = S(r) - m * z;
+      fm[r] = m;
+      frb[r] = srb[r] = rb;
+      fcrb[r] = scrb[r] = c * rb;
+      sv[r] = v;
+      sz[r] = z;
+    }
+    v *= rb;
+    z *= rb;
+    T w = -scrb[r0 + M - 2];
+    fv[r0 + M - 2] = v;
+    fw[r0 + M - 2] = w;
+    fz[r0 + M - 2] = z;
+    vL = v;
+    zL = z;
+#pragma unroll 8
+    for (int k = M - 3; k >= 0; --k) {
+      const int r = r0 + k;
+      const T crb = scrb[r], rbk = srb[r];
+      v = sv[r] * rbk - crb * v;
+      z = sz[r] * rbk - crb * z;
+      w = -crb * w;
+      fv[r] = v;
+      fw[r] = w;
+      fz[r] = z;
+    }
+    w0 = w;
+    z0 = z;
+    aL = A(r0 + M - 1);
+    cL = C(r0 + M - 1);
+    sL = S(r0 + M - 1);
+    fac_seg(f, kSaL)[q] = aL;
+    fac_seg(f, kScL)[q] = cL;
+    fac_seg(f, kSsL)[q] = sL;
+  }
+  // Reduced system: sub- and super-diagonal A = aL*v_{M-2} <= 0 and
+  // C = cL*w'_0 <= 0 (w' of the next segment), row sum
+  // delta = sL - aL*z_{M-2} - cL*z'_0 >= 0; its pivots in g-form.
+  const T w0n = __shfl(w0, q + 1), z0n = __shfl(z0, q + 1);
+  const bool last = q + 1 >= P;
+  const T Aq = aL * vL, Cq = last ? T(0) : cL * w0n;
+  const T delta = sL - aL * zL - (last ? T(0) : cL * z0n);
+  T gq = T(0), rrb = T(0);
+  for (int j = 0; j < P; ++j) {  // serial over the segments, in every lane
+    const T Aj = __shfl(Aq, j), Cj = __shfl(Cq, j), dj = __shfl(delta, j);
+    const T m = j == 0 ? T(0) : Aj * rrb;
+    gq = j == 0 ? dj - Aj : dj - m * gq;
+    rrb = rcp(gq - Cj);
+    if (q == 0) {
+      fac_seg(f, kSrm)[j] = m;
+      fac_seg(f, kSrrb)[j] = rrb;
+      fac_seg(f, kSrC)[j] = Cj;
+    }
+  }
+}
+
+template <typename T, int M>
+struct ColLds {
+  static constexpr int kRows = kColWaves * M;
+  T m[kRows], rb[kRows], crb[kRows], v[kRows], w[kRows];  // per row
+  double a64[kRows], c64[kRows];  // the taps in FP64 for the residual
+  T aL[kColWaves], cL[kColWaves];                  // each segment's last row
+  T rm[kColWaves], rrb[kColWaves], rC[kColWaves];  // reduced system's factor
+  T y0[2][LVC_COL_BLOCK], Dt[2][LVC_COL_BLOCK];    // exchange, one per solve
+};
+
+// Solves in place for the right-hand side held in x (this lane's segment p of
+// one column), using exchange slot `sl`.  Returns the neighbouring unknowns
+// XL = x_{-1} (previous segment's last) and XN = x_M (next segment's first).
+template <typename T, int M>
+__device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int P,
+                                          int gbase, int lane, int sl, T &XL, T &XN) {
+  const int r0 = p * M;
+  const T dlast = x[M - 1];
+#pragma unroll
+  for (int k = 1; k < M - 1; ++k) x[k] -= L.m[r0 + k] * x[k - 1];
+  x[M - 2] *= L.rb[r0 + M - 2];
+#pragma unroll
+  for (int k = M - 3; k >= 0; --k) x[k] = x[k] * L.rb[r0 + k] - L.crb[r0 + k] * x[k + 1];
+  const int me = (gbase + p) * 64 + lane;
+  L.y0[sl][me] = x[0];
+  L.Dt[sl][me] = dlast - L.aL[p] * x[M - 2];
+  __syncthreads();
+  // Reduced system of the column: Thomas over its P segments.  (Waves past
+  // the last column group compute on wrapped slots and discard the result.)
+  T Yr[kColWaves], y0n = T(0);
+  T prev = T(0);
+#pragma unroll
+  for (int q = 0; q < kColWaves; ++q) {
+    if (q < P) {
+      const int at = ((gbase + q) & (kColWaves - 1)) * 64 + lane;
+      const int an = ((gbase + q + 1) & (kColWaves - 1)) * 64 + lane;
+      const T yn = q + 1 < P ? L.y0[sl][an] : T(0);
+      if (q == p) y0n = yn;
+      const T D = L.Dt[sl][at] - L.cL[q] * yn;
+      prev = q == 0 ? D : D - L.rm[q] * prev;
+      Yr[q] = prev;
+    }
+  }
+  T Xm = T(0), Xc = T(0), Xp = T(0), nxt = T(0);
+#pragma unroll
+  for (int q = kColWaves - 1; q >= 0; --q) {
+    if (q < P) {
+      nxt = (Yr[q] - L.rC[q] * nxt) * L.rrb[q];
+      if (q == p + 1) Xp = nxt;
+      if (q == p) Xc = nxt;
+      if (q == p - 1) Xm = nxt;
+    }
+  }
+  // x_k = y_k + v_k X_{p-1} + w_k X_p; the next segment's first unknown is
+  // formed by the same fma sequence, so XN is bit-identical to the value that
+  // segment holds (the refinement's residual needs the exact neighbour).
+#pragma unroll
+  for (int k = 0; k < M - 1; ++k)
+    x[k] = fma(L.w[r0 + k], Xc, fma(L.v[r0 + k], Xm, x[k]));
+  x[M - 1] = Xc;
+  XL = Xm;
+  XN = p + 1 < P ? fma(L.w[r0 + M], Xp, fma(L.v[r0 + M], Xc, y0n)) : T(0);
+}
+
+template <typename T, int M, bool REF>
+__global__ __launch_bounds__(LVC_COL_BLOCK) void lvc_ycol_kernel(
+    lvc_problem *probs, const lvc_tile *__restrict__ tiles,
+    const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R,
+    const T *__restrict__ yfac) {
+  __shared__ ColLds<T, M> L;
+  const lvc_tile tile = tiles[blockIdx.x];
+  lvc_problem &pr = probs[tile.problem];
+  const int kt = pr.k;
+  if (kt < 0) return;  // problem finished: whole workgroup exits
+  const int nx = pr.nx, ny = pr.ny;
+  const int P = tile.lanes, groups = tile.cpl;
+  if (P < 1 || groups < 1 || P * groups > kColWaves) return;  // not a column tile
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int grp = wave / P, p = wave - grp * P;
+  const bool used = grp < groups;  // waves past the last group only synchronise
+  const int cidx = grp * 64 + lane;
+  const bool active = used && cidx < tile.count;
+  const int col = tile.first + cidx;
+  const int r0 = p * M;
+  const double *g = geo + pr.geo_off;
+
+  // This lane's right-hand side (the loads overlap the factor-table copy
+  // below).  Idle lanes read a valid column and discard it.
+  const bool full = r0 + M <= ny;  // wave-uniform
+  T d[M];
+  const T *src = Y + pr.state_off + (active ? col : tile.first) + (int64_t)r0 * nx;
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) d[k] = src[(int64_t)k * nx];
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; ++k) d[k] = r0 + k < ny ? src[(int64_t)k * nx] : T(0);
+  }
+  if (!active) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) d[k] = T(0);
+  }
+
+  // The step's factor table (written by the x sweep) into LDS.
+  const T *f = yfac + (int64_t)tile.problem * LVC_YFAC_LEN;
+  for (int r = threadIdx.x; r < P * M; r += LVC_COL_BLOCK) {
+    L.m[r] = f[kFm * LVC_COL_ROWS + r];
+    L.rb[r] = f[kFrb * LVC_COL_ROWS + r];
+    L.crb[r] = f[kFcrb * LVC_COL_ROWS + r];
+    L.v[r] = f[kFv * LVC_COL_ROWS + r];
+    L.w[r] = f[kFw * LVC_COL_ROWS + r];
+  }
+  if (threadIdx.x < P) {
+    const T *fs = f + 6 * LVC_COL_ROWS;
+    L.aL[threadIdx.x] = fs[kSaL * kColWaves + threadIdx.x];
+    L.cL[threadIdx.x] = fs[kScL * kColWaves + threadIdx.x];
+    L.rm[threadIdx.x] = fs[kSrm * kColWaves + threadIdx.x];
+    L.rrb[threadIdx.x] = fs[kSrrb * kColWaves + threadIdx.x];
+    L.rC[threadIdx.x] = fs[kSrC * kColWaves + threadIdx.x];
+  }
+  if (REF) {
+    // The residual uses the same (T-rounded) taps as the explicit y term of
+    // the x sweep, so that the refined solve stays consistent with it.
+    const Taps<T> tp(g, nx, ny);
+    for (int r = threadIdx.x; r < P * M; r += LVC_COL_BLOCK) {
+      L.a64[r] = r < ny ? (double)tp.ay[r] : 0.0;
+      L.c64[r] = r < ny ? (double)tp.cy[r] : 0.0;
+    }
+  }
+  __syncthreads();
+
+  const int gbase = grp * P;
+  T x[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) x[k] = d[k];
+  T XL, XN;
+  col_solve<T, M>(x, L, p, P, gbase, lane, 0, XL, XN);
+  if (REF) {
+    // Residual in FP64 (T-rounded taps, as above), solved again with the same
+    // factorisation; the correction is added in T (one rounding of the
+    // exact sum x + e, as an FP64 addition rounded to T would give).
+    const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
+    const double s64 = (double)(T)(1.0 / (tnext - tnow));  // dt as the x sweep rounds it
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double xk = (double)x[k];
+      const double xm = (double)(k > 0 ? x[k - 1] : XL);
+      const double xp = (double)(k < M - 1 ? x[k + 1] : XN);
+      const double sk = r0 + k < ny ? s64 : 1.0;
+      const double ax = sk * xk + L.a64[r0 + k] * (xm - xk) + L.c64[r0 + k] * (xp - xk);
+      d[k] = (T)((double)d[k] - ax);
+    }
+    T EL, EN;
+    col_solve<T, M>(d, L, p, P, gbase, lane, 1, EL, EN);
+#pragma unroll
+    for (int k = 0; k < M; ++k) x[k] += d[k];
+  }
+  T *dst = R + pr.state_off + col + (int64_t)r0 * nx;
+  if (active) {
+    if (full) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) dst[(int64_t)k * nx] = x[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+        if (r0 + k < ny) dst[(int64_t)k * nx] = x[k];
+    }
+  }
+  // The last of the problem's workgroups to finish advances its time index
+  // (every workgroup of the problem read `k` before its first barrier).
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(&pr.ydone, 1) + 1;
+    if (done == pr.ytiles) {
+      pr.ydone = 0;
+      pr.k = kt - 1;
+    }
+  }
+}
+
 // value (:236): the price at (yindex, xindex).
 template <typename T, typename O>
 __global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
@@ -609,38 +965,64 @@ __global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
 template <typename T, int M, bool REF>
 int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
                   const lvc_tile *tiles, int ntiles, const double *geo,
-                  const void *in, void *out, int lds_bytes) {
+                  const void *in, void *out, int lds_bytes, void *yfac, int yseg,
+                  const int32_t *fprobs, int nfac) {
   if (ntiles <= 0) return 0;
-  if (xs) {
-    lvc_xsweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, 0, st>>>(
-        probs, tiles, geo, (const T *)in, (T *)out);
+  if constexpr (M >= 32) {
+    if (xs || !yfac) return (int)hipErrorInvalidValue;  // column segments: y sweep only
+    lvc_ycol_kernel<T, M, REF><<<ntiles, LVC_COL_BLOCK, 0, st>>>(
+        const_cast<lvc_problem *>(probs), tiles, geo, (const T *)in, (T *)out,
+        (const T *)yfac);
+    return (int)hipGetLastError();
   } else {
-    if (lds_bytes <= 0 || lds_bytes > 65536) return (int)hipErrorInvalidValue;
-    lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, lds_bytes, st>>>(
-        const_cast<lvc_problem *>(probs), tiles, geo, (const T *)in, (T *)out);
+    if (xs) {
+      if (yseg > 0 && (!yfac || !fprobs || !lvc_seg_is_col(yseg))) return (int)hipErrorInvalidValue;
+      if (yseg <= 0 || nfac < 0) nfac = 0;
+      lvc_xsweep_kernel<T, M, REF><<<ntiles + nfac, LVC_BLOCK, 0, st>>>(
+          probs, tiles, geo, (const T *)in, (T *)out, (T *)yfac, yseg, fprobs, nfac);
+    } else {
+      if (lds_bytes <= 0 || lds_bytes > 65536) return (int)hipErrorInvalidValue;
+      lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, lds_bytes, st>>>(
+          const_cast<lvc_problem *>(probs), tiles, geo, (const T *)in, (T *)out);
+    }
+    return (int)hipGetLastError();
   }
-  return (int)hipGetLastError();
 }
 
 template <typename T, bool REF>
 int dispatch_seg(hipStream_t st, bool xs, int seg, const lvc_problem *probs,
                  const lvc_tile *tiles, int ntiles, const double *geo,
-                 const void *in, void *out, int lds) {
+                 const void *in, void *out, int lds, void *yfac, int yseg,
+                 const int32_t *fprobs, int nfac) {
+#define LVC_SEG_CASE(S)                                                         \
+  case S:                                                                      \
+    return launch_sweeps<T, S, REF>(st, xs, probs, tiles, ntiles, geo, in, out, \
+                                    lds, yfac, yseg, fprobs, nfac);
   switch (seg) {
-    case 4: return launch_sweeps<T, 4, REF>(st, xs, probs, tiles, ntiles, geo, in, out, lds);
-    case 8: return launch_sweeps<T, 8, REF>(st, xs, probs, tiles, ntiles, geo, in, out, lds);
-    case 16: return launch_sweeps<T, 16, REF>(st, xs, probs, tiles, ntiles, geo, in, out, lds);
+    LVC_SEG_CASE(4)
+    LVC_SEG_CASE(8)
+    LVC_SEG_CASE(16)
+    LVC_SEG_CASE(32)
+    LVC_SEG_CASE(64)
     default: return (int)hipErrorInvalidValue;
   }
+#undef LVC_SEG_CASE
 }
 
 int dispatch(hipStream_t st, bool xs, int precision, int seg,
              const lvc_problem *probs, const lvc_tile *tiles, int ntiles,
-             const double *geo, const void *in, void *out, int lds) {
+             const double *geo, const void *in, void *out, int lds, void *yfac,
+             int yseg, const int32_t *fprobs, int nfac) {
   switch (precision) {
-    case LVC_FP32: return dispatch_seg<float, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out, lds);
-    case LVC_FP32_REFINE: return dispatch_seg<float, true>(st, xs, seg, probs, tiles, ntiles, geo, in, out, lds);
-    case LVC_FP64: return dispatch_seg<double, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out, lds);
+    case LVC_FP32:
+      return dispatch_seg<float, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out,
+                                        lds, yfac, yseg, fprobs, nfac);
+    case LVC_FP32_REFINE:
+      return dispatch_seg<float, true>(st, xs, seg, probs, tiles, ntiles, geo, in, out,
+                                       lds, yfac, yseg, fprobs, nfac);
+    case LVC_FP64:
+      return dispatch_seg<double, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out,
+                                         lds, yfac, yseg, fprobs, nfac);
     default: return (int)hipErrorInvalidValue;
   }
 }
@@ -649,7 +1031,9 @@ int dispatch(hipStream_t st, bool xs, int precision, int seg,
 
 extern "C" {
 
-int lvc_seg_supported(int seg) { return seg == 4 || seg == 8 || seg == 16; }
+int lvc_seg_supported(int seg, int ysweep) {
+  return seg == 4 || seg == 8 || seg == 16 || (ysweep && (seg == 32 || seg == 64));
+}
 
 int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
                      double *geo, void *R) {
@@ -665,17 +1049,18 @@ int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
 
 int lvc_launch_xsweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
-                      int ntiles, const double *geo, const void *R, void *Y) {
+                      int ntiles, const double *geo, const void *R, void *Y,
+                      void *yfac, int yseg, const int32_t *fprobs, int nfac) {
   return dispatch((hipStream_t)stream, true, precision, seg, probs, tiles,
-                  ntiles, geo, R, Y, 0);
+                  ntiles, geo, R, Y, 0, yfac, yseg, fprobs, nfac);
 }
 
 int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *Y, void *R,
-                      int lds_bytes) {
+                      int lds_bytes, const void *yfac) {
   return dispatch((hipStream_t)stream, false, precision, seg, probs, tiles,
-                  ntiles, geo, Y, R, lds_bytes);
+                  ntiles, geo, Y, R, lds_bytes, const_cast<void *>(yfac), 0, nullptr, 0);
 }
 
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,

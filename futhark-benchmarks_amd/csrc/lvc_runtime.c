@@ -122,7 +122,7 @@ int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
     return 0;
   }
   if (strcmp(name, "lvc.seg_x") == 0 || strcmp(name, "lvc.seg_y") == 0) {
-    if (v != 0 && !lvc_seg_supported((int)v))
+    if (v != 0 && !lvc_seg_supported((int)v, name[8] == 'y'))
       return 1;
     if (name[8] == 'x')
       cfg->seg_x = (int)v;

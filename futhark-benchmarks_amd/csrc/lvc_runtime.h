@@ -121,6 +121,10 @@ typedef struct lvc_batch {
   int *x_active, *y_active; /* [steps] */
   int graph;
   int ylds; /* LDS bytes of the y-sweep launches */
+  void *d_yfac; /* y factor tables (column y sweep only), LVC_YFAC_LEN each */
+  size_t yfac_bytes;
+  int32_t *d_fprobs; /* problems by numT, longest first (factor workgroups) */
+  int *f_active;     /* [steps] problems still running at step s */
 } lvc_batch;
 
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,

@@ -51,6 +51,9 @@ struct lvc_server {
   lvc_problem *d_slots;
   double *d_geo;
   void *d_R, *d_Y;
+  void *d_yfac; /* y factor tables per slot (column y sweep only) */
+  int32_t *h_fprobs, *d_fprobs; /* resident slots (factor workgroups) */
+  int nfac;
   double *d_out;
   double *h_out;
   /* tiles of the resident set */
@@ -94,13 +97,14 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
   lvc_choose_segs(ctx, precision, 256, 256, &s->seg_x, &s->seg_y);
   if (lvc_lanes_for(SLOT_MAX_N, s->seg_x) > 64)
     s->seg_x = 8;
-  if (lvc_lanes_for(SLOT_MAX_N, s->seg_y) > 64)
-    s->seg_y = 8;
+  if (!lvc_yseg_fits(SLOT_MAX_N, s->seg_y))
+    s->seg_y = lvc_seg_is_col(s->seg_y) ? 64 : 8;
   s->h_slots = calloc((size_t)max_slots, sizeof(lvc_problem));
   s->slot_id = malloc((size_t)max_slots * sizeof(int64_t));
   s->steps_left = calloc((size_t)max_slots, sizeof(int));
   s->h_out = calloc((size_t)max_slots, sizeof(double));
-  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_out;
+  s->h_fprobs = calloc((size_t)max_slots, sizeof(int32_t));
+  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_out || !s->h_fprobs;
   if (rc)
     lvc_set_error(ctx, "host allocation failed");
   for (int i = 0; !rc && i < max_slots; i++) {
@@ -112,7 +116,10 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
          lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_geo * sizeof(double), (void **)&s->d_geo) ||
          lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_elems * s->real, &s->d_R) ||
          lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_elems * s->real, &s->d_Y) ||
-         lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(double), (void **)&s->d_out);
+         lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(double), (void **)&s->d_out) ||
+         (lvc_seg_is_col(s->seg_y) &&
+          (lvc_dev_alloc(ctx, (size_t)max_slots * LVC_YFAC_LEN * s->real, &s->d_yfac) ||
+           lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(int32_t), (void **)&s->d_fprobs)));
   if (!rc)
     rc = lvc_upload(ctx, s->d_slots, s->h_slots, (size_t)max_slots * sizeof(lvc_problem));
   pthread_mutex_unlock(&ctx->lock);
@@ -134,6 +141,10 @@ void lvc_server_free(struct lvc_server *s) {
   lvc_dev_free(ctx, s->d_R, (size_t)s->max_slots * s->slot_elems * s->real);
   lvc_dev_free(ctx, s->d_Y, (size_t)s->max_slots * s->slot_elems * s->real);
   lvc_dev_free(ctx, s->d_out, (size_t)s->max_slots * sizeof(double));
+  if (s->d_yfac)
+    lvc_dev_free(ctx, s->d_yfac, (size_t)s->max_slots * LVC_YFAC_LEN * s->real);
+  if (s->d_fprobs)
+    lvc_dev_free(ctx, s->d_fprobs, (size_t)s->max_slots * sizeof(int32_t));
   if (s->d_tiles)
     lvc_dev_free(ctx, s->d_tiles, (size_t)s->tiles_cap * sizeof(lvc_tile));
   pthread_mutex_unlock(&ctx->lock);
@@ -141,6 +152,7 @@ void lvc_server_free(struct lvc_server *s) {
   free(s->slot_id);
   free(s->steps_left);
   free(s->h_out);
+  free(s->h_fprobs);
   free(s->h_tiles);
   free(s->queue);
   free(s->results);
@@ -158,7 +170,7 @@ int64_t lvc_server_submit(struct lvc_server *s, const struct lvc_request *r) {
                   SLOT_MAX_N);
     return -1;
   }
-  if (lvc_lanes_for(r->numX, s->seg_x) > 64 || lvc_lanes_for(r->numY, s->seg_y) > 64) {
+  if (lvc_lanes_for(r->numX, s->seg_x) > 64 || !lvc_yseg_fits(r->numY, s->seg_y)) {
     lvc_set_error(ctx, "grid too large for the server's segment lengths");
     return -1;
   }
@@ -228,10 +240,12 @@ static int rebuild_tiles(struct lvc_server *s) {
   }
   int tx = 0, ty = nx_t;
   s->ylds = 0;
+  s->nfac = 0;
   for (int i = 0; i < s->max_slots; i++) {
     if (s->slot_id[i] < 0)
       continue;
     const lvc_problem *q = &s->h_slots[i];
+    s->h_fprobs[s->nfac++] = i;
     const int gx = lvc_lanes_for(q->nx, s->seg_x);
     const int rows = LVC_BLOCK / gx;
     int gy, cpl, cols;
@@ -251,6 +265,9 @@ static int rebuild_tiles(struct lvc_server *s) {
   s->nxt = nx_t;
   s->nyt = ny_t;
   s->tiles_dirty = 0;
+  if (s->d_fprobs && s->nfac &&
+      lvc_upload(ctx, s->d_fprobs, s->h_fprobs, (size_t)s->nfac * sizeof(int32_t)))
+    return 1;
   return need ? lvc_upload(ctx, s->d_tiles, s->h_tiles, (size_t)need * sizeof(lvc_tile)) : 0;
 }
 
@@ -314,14 +331,16 @@ int32_t lvc_server_step(struct lvc_server *s) {
     void *st = (void *)ctx->stream;
     int tok = lvc_prof_begin(ctx, "lvc_xsweep");
     int e = lvc_launch_xsweep(st, s->precision, s->seg_x, s->d_slots, s->d_tiles,
-                              s->nxt, s->d_geo, s->d_R, s->d_Y);
+                              s->nxt, s->d_geo, s->d_R, s->d_Y, s->d_yfac,
+                              lvc_seg_is_col(s->seg_y) ? s->seg_y : 0, s->d_fprobs,
+                              s->nfac);
     lvc_prof_end(ctx, tok);
     rc = lvc_check_hip(ctx, e, "x sweep kernel");
     if (!rc) {
       tok = lvc_prof_begin(ctx, "lvc_ysweep");
       e = lvc_launch_ysweep(st, s->precision, s->seg_y, s->d_slots,
                             s->d_tiles + s->nxt, s->nyt, s->d_geo, s->d_Y, s->d_R,
-                            s->ylds);
+                            s->ylds, s->d_yfac);
       lvc_prof_end(ctx, tok);
       rc = lvc_check_hip(ctx, e, "y sweep kernel");
     }

@@ -49,8 +49,8 @@ void lvc_tune_load(struct futhark_context *ctx) {
     return;
   int p, nx, ny, sx, sy;
   while (fscanf(f, "%d %d %d %d %d", &p, &nx, &ny, &sx, &sy) == 5)
-    if (p >= LVC_FP32 && p <= LVC_FP64 && lvc_seg_supported(sx) &&
-        lvc_seg_supported(sy))
+    if (p >= LVC_FP32 && p <= LVC_FP64 && lvc_seg_supported(sx, 0) &&
+        lvc_seg_supported(sy, 1))
       table_put(ctx, p, nx, ny, sx, sy);
   fclose(f);
 }
@@ -67,22 +67,23 @@ static void cache_append(struct futhark_context *ctx, int precision, int nx,
 }
 
 /* Built-in choice: the fastest pair measured on MI355X at 256x256 (128
- * options, 63 steps): FP32+refinement 8/4 (4.04 ms; 8/8 4.63, 4/4 4.61),
- * FP32 8/8 (3.09 ms), FP64 4/4 (5.76 ms; 8/8 6.44).  x segments of 8 keep a
- * 256-point row on 32 lanes; y segments of 4 put a column on 64 lanes, each
- * solving 8 columns with one factorisation. */
+ * options, 63 steps), per call: FP32+refinement x 8 / y 64 (3.69 ms; 8/32
+ * 4.01, 8/4 with PCR y 4.22), FP32 8/32 (2.60 ms; 8/8 3.24), FP64 4/32
+ * (4.82 ms; 4/4 5.93).  x segments of 8 keep a 256-point row on 32 lanes; the
+ * column y sweep (segments 32, 64) factors the shared y matrix once per step.
+ * Grids taller than the column sweep takes (8 segments) fall back to 16. */
 static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
   (void)nx;
   (void)ny;
   if (precision == LVC_FP64) {
     *sx = 4;
-    *sy = 4;
+    *sy = 32;
   } else if (precision == LVC_FP32) {
     *sx = 8;
-    *sy = 8;
+    *sy = 32;
   } else {
     *sx = 8;
-    *sy = 4;
+    *sy = 64;
   }
 }
 
@@ -141,12 +142,12 @@ void lvc_tune_pick(struct futhark_context *ctx, int precision, int nx, int ny,
   builtin(precision, nx, ny, seg_x, seg_y);
   if (!ctx->cfg || !ctx->cfg->autotune || !ctx->ok)
     return;
-  static const int segs[] = {4, 8, 16};
+  static const int segs[] = {4, 8, 16, 32, 64};
   double best = -1;
   int bx = *seg_x, by = *seg_y;
   for (int a = 0; a < 3; a++)
-    for (int c = 0; c < 3; c++) {
-      if (lvc_lanes_for(nx, segs[a]) > 64 || lvc_lanes_for(ny, segs[c]) > 64)
+    for (int c = 0; c < 5; c++) {
+      if (lvc_lanes_for(nx, segs[a]) > 64 || !lvc_yseg_fits(ny, segs[c]))
         continue;
       double ms = time_candidate(ctx, precision, nx, ny, segs[a], segs[c]);
       if (ms > 0 && (best < 0 || ms < best)) {

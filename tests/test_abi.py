@@ -54,6 +54,11 @@ def test_config_and_tuning_params():
     assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.precision", 7) != 0
     assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_x", 3) != 0
     assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_x", 8) == 0
+    # column segments (one lane per segment) exist for the y sweep only
+    assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_x", 32) != 0
+    for seg in (4, 8, 16, 32, 64):
+        assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_y", seg) == 0
+    assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_y", 128) != 0
     L.futhark_context_config_set_device(cfg, b"hip:0")
     L.futhark_context_config_set_cache_file(cfg, b"/tmp/lvc-tune-cache.txt")
     L.futhark_context_config_free(cfg)

@@ -210,6 +210,51 @@ def test_batch_log_hazard_matches_oracle(gpu_ctx):
         assert rel_err(got[fin], ref[fin]) < F64_REL
 
 
+# Ragged shapes for the y-sweep variants: ny not a multiple of the segment,
+# a single segment per column (plain Thomas), nx below and above one 64-column
+# group, the 3x3 minimum.
+Y_SHAPES_SMALL = [(37, 200, 5), (64, 64, 4), (100, 33, 4), (129, 130, 4), (256, 256, 3),
+                  (3, 3, 3), (70, 31, 5), (200, 97, 3)]
+Y_SHAPES_TALL = [(70, 257, 3), (16, 500, 3), (130, 384, 3), (40, 512, 2)]
+
+
+def _shapes_problems(shapes, seed):
+    rng = np.random.default_rng(seed)
+    n = len(shapes)
+    return dict(numX=np.array([s[0] for s in shapes], np.int32),
+                numY=np.array([s[1] for s in shapes], np.int32),
+                numT=np.array([s[2] for s in shapes], np.int32),
+                s0=rng.uniform(0.02, 0.05, n), strike=rng.uniform(0.0, 0.05, n),
+                t=rng.uniform(1, 10, n), alpha=rng.uniform(0.1, 0.3, n),
+                nu=rng.uniform(0.4, 0.8, n), beta=rng.uniform(0.3, 0.7, n))
+
+
+@pytest.mark.parametrize("seg_y", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("shapes", ["small", "tall"])
+@pytest.mark.parametrize("precision", [lvc.PRECISION_FP64, lvc.PRECISION_FP32_REFINE])
+def test_y_segments_vs_oracle(seg_y, shapes, precision):
+    """Every y-sweep variant (lane segments 4/8/16 with PCR, column segments
+    32/64 with Thomas) against the FP64 oracle on ragged shapes."""
+    say(f"case: y segment {seg_y}, {shapes} shapes, precision {precision}")
+    pr = _shapes_problems(Y_SHAPES_SMALL if shapes == "small" else Y_SHAPES_TALL,
+                          seed=seg_y)
+    with lvc.LocVolCalib(tuning={"lvc.seg_y": seg_y, "lvc.seg_x": 8}) as ctx:
+        got = ctx.main_batch(**pr, precision=precision)
+    ref = oracle_batch(pr)
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(got), fin)
+    g, r = got[fin], ref[fin]
+    if precision == lvc.PRECISION_FP64:
+        # relative per price; a zero price (deep out of the money) relative to
+        # the largest price of the batch
+        den = np.where(r != 0, np.abs(r), np.max(np.abs(r)))
+        assert np.max(np.abs(g - r) / den) < F64_REL
+    else:
+        err = np.max(np.abs(g - r)) / np.max(np.abs(r))
+        say(f"   max err / max price {err:.3e}")
+        assert err < F32R_REL
+
+
 def test_empty_and_errors(gpu_ctx):
     say("case: outer=0 and invalid sizes")
     assert gpu_ctx.main_f64(0, 32, 32, 8, *CFG.values()).shape == (0,)
