@@ -40,7 +40,8 @@
 // solution is formed in FP64 (with the same taps and the same rounded time
 // step dtInv that the explicit part and the combination use, so that the
 // scheme stays consistent), solved again in FP32 with the same
-// factorisation, and added in FP64.
+// factorisation, and added with one rounding (x + e in FP32 is the correctly
+// rounded sum).
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -393,7 +394,7 @@ struct SegSys {
 
 // One refinement pass on the solution x of  s*x + a*(x[-1]-x) + c*(x[+1]-x) = d:
 // the residual in FP64, a second solve with the same factorisation, the
-// correction added in FP64.
+// correction added with one rounding.
 template <typename T, int M>
 __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
                                        const T (&a)[M], const T (&c)[M],
@@ -411,8 +412,11 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
   }
   T e[M];
   sys.solve(r, e);
+  // x + e rounded once to T: the correctly rounded sum, which is what an FP64
+  // addition rounded back to T gives (the exact sum of two floats fits a
+  // double whenever their exponents are within 29 of each other).
 #pragma unroll
-  for (int k = 0; k < M; ++k) x[k] = (T)((double)x[k] + (double)e[k]);
+  for (int k = 0; k < M; ++k) x[k] += e[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -510,8 +514,8 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
     const lvc_problem *__restrict__ probs, const lvc_tile *__restrict__ tiles,
     const double *__restrict__ geo, const T *__restrict__ R, T *__restrict__ Y,
     T *__restrict__ yfac, int yseg, const int32_t *__restrict__ fprobs, int nfac) {
-  __shared__ T smem[kXsweepLds<T, M>];
-  __shared__ T fscr[kFacScratch];  // scratch of the factor workgroups
+  // row tiles: U and V panels; factor workgroups: the factor scratch
+  __shared__ T smem[kXsweepLds<T, M> > kFacScratch ? kXsweepLds<T, M> : kFacScratch];
   if ((int)blockIdx.x < nfac) {
     if (yseg > 0 && threadIdx.x < 64) {
       const int p = fprobs[blockIdx.x];
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
         __builtin_amdgcn_s_setprio(3);
         const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
         ycol_factor<T>(pr, geo + pr.geo_off, (T)(1.0 / (tnext - tnow)), yseg,
-                       yfac + (int64_t)p * LVC_YFAC_LEN, fscr);
+                       yfac + (int64_t)p * LVC_YFAC_LEN, smem);
         __builtin_amdgcn_s_setprio(0);
       }
     }
