@@ -799,16 +799,15 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
   L.y0[sl][me] = x[0];
   L.Dt[sl][me] = dlast - L.aL[p] * x[M - 2];
   __syncthreads();
-  // Reduced system of the column: Thomas over its P segments.  (Waves past
-  // the last column group compute on wrapped slots and discard the result.)
+  // Reduced system of the column: Thomas over its P segments (slots
+  // gbase .. gbase+P-1, all below kColWaves).
   T Yr[kColWaves], y0n = T(0);
   T prev = T(0);
 #pragma unroll
   for (int q = 0; q < kColWaves; ++q) {
     if (q < P) {
-      const int at = ((gbase + q) & (kColWaves - 1)) * 64 + lane;
-      const int an = ((gbase + q + 1) & (kColWaves - 1)) * 64 + lane;
-      const T yn = q + 1 < P ? L.y0[sl][an] : T(0);
+      const int at = (gbase + q) * 64 + lane;
+      const T yn = q + 1 < P ? L.y0[sl][at + 64] : T(0);
       if (q == p) y0n = yn;
       const T D = L.Dt[sl][at] - L.cL[q] * yn;
       prev = q == 0 ? D : D - L.rm[q] * prev;
@@ -836,8 +835,10 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
   XN = p + 1 < P ? fma(L.w[r0 + M], Xp, fma(L.v[r0 + M], Xc, y0n)) : T(0);
 }
 
+// Segments of 32 rows: at least 4 waves per SIMD (two workgroups per CU), so
+// that the 8 segments x 64 columns tiles of 256-row columns run in one round.
 template <typename T, int M, bool REF>
-__global__ __launch_bounds__(LVC_COL_BLOCK) void lvc_ycol_kernel(
+__global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kernel(
     lvc_problem *probs, const lvc_tile *__restrict__ tiles,
     const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R,
     const T *__restrict__ yfac) {
@@ -852,9 +853,14 @@ __global__ __launch_bounds__(LVC_COL_BLOCK) void lvc_ycol_kernel(
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int grp = wave / P, p = wave - grp * P;
-  const bool used = grp < groups;  // waves past the last group only synchronise
+  // Waves of column groups past the tile's columns leave before the first
+  // barrier (a barrier waits only for the waves still running); the rest are
+  // threads 0 .. nact-1.
+  const int gused = min(groups, (tile.count + 63) / 64);
+  if (grp >= gused) return;
+  const int nact = gused * P * 64;
   const int cidx = grp * 64 + lane;
-  const bool active = used && cidx < tile.count;
+  const bool active = cidx < tile.count;
   const int col = tile.first + cidx;
   const int r0 = p * M;
   const double *g = geo + pr.geo_off;
@@ -878,7 +884,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK) void lvc_ycol_kernel(
 
   // The step's factor table (written by the x sweep) into LDS.
   const T *f = yfac + (int64_t)tile.problem * LVC_YFAC_LEN;
-  for (int r = threadIdx.x; r < P * M; r += LVC_COL_BLOCK) {
+  for (int r = threadIdx.x; r < P * M; r += nact) {
     L.m[r] = f[kFm * LVC_COL_ROWS + r];
     L.rb[r] = f[kFrb * LVC_COL_ROWS + r];
     L.crb[r] = f[kFcrb * LVC_COL_ROWS + r];
@@ -897,7 +903,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK) void lvc_ycol_kernel(
     // The residual uses the same (T-rounded) taps as the explicit y term of
     // the x sweep, so that the refined solve stays consistent with it.
     const Taps<T> tp(g, nx, ny);
-    for (int r = threadIdx.x; r < P * M; r += LVC_COL_BLOCK) {
+    for (int r = threadIdx.x; r < P * M; r += nact) {
       L.a64[r] = r < ny ? (double)tp.ay[r] : 0.0;
       L.c64[r] = r < ny ? (double)tp.cy[r] : 0.0;
     }

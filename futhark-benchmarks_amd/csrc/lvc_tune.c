@@ -67,11 +67,13 @@ static void cache_append(struct futhark_context *ctx, int precision, int nx,
 }
 
 /* Built-in choice: the fastest pair measured on MI355X at 256x256 (128
- * options, 63 steps), per call: FP32+refinement x 8 / y 64 (3.69 ms; 8/32
- * 4.01, 8/4 with PCR y 4.22), FP32 8/32 (2.60 ms; 8/8 3.24), FP64 4/32
- * (4.82 ms; 4/4 5.93).  x segments of 8 keep a 256-point row on 32 lanes; the
- * column y sweep (segments 32, 64) factors the shared y matrix once per step.
- * Grids taller than the column sweep takes (8 segments) fall back to 16. */
+ * options, 63 steps), per call: FP32+refinement x 8 / y 32 (3.60 ms; 8/64
+ * 3.86, 8/4 with PCR y 4.22), FP32 8/32 (2.47 ms; 8/8 3.24), FP64 4/32
+ * (4.96 ms; 4/4 5.93); also the fastest on the 512-problem uneven batch
+ * (FP32+refinement 11.4 ms vs 14.3 for 8/8).  x segments of 8 keep a
+ * 256-point row on 32 lanes; the column y sweep (segments 32, 64) factors the
+ * shared y matrix once per step.  Columns taller than 8 segments take 64, then
+ * fall back to the lane segments of 16. */
 static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
   (void)nx;
   (void)ny;
@@ -83,7 +85,7 @@ static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
     *sy = 32;
   } else {
     *sx = 8;
-    *sy = 64;
+    *sy = 32;
   }
 }
 
