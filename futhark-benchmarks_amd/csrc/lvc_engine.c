@@ -81,6 +81,23 @@ static void nt_sort_desc(int *order, int n, const lvc_problem *probs) {
   }
 }
 
+/* Problems per x tile sharing a factorisation: lvc.xshare when set, else
+ * the measured best at 128 x 256x256x64 (FP32+refinement and FP32: 8, 38.0K
+ * and 58.8K options/s vs 31.5K and 43.0K unshared; FP64: 16, 36.8K vs 23.0K). */
+int lvc_xshare(struct futhark_context *ctx, int precision) {
+  if (ctx->cfg && ctx->cfg->xshare > 0)
+    return ctx->cfg->xshare;
+  return precision == LVC_FP64 ? 16 : 8;
+}
+
+/* Problems whose factorisations coincide: everything but the strike (and
+ * the state) equal.  The reference's main is a batch of such problems
+ * (LocVolCalib.fut:244-251). */
+static int same_factor(const lvc_problem *a, const lvc_problem *b) {
+  return a->nx == b->nx && a->ny == b->ny && a->nt == b->nt && a->s0 == b->s0 &&
+         a->t == b->t && a->alpha == b->alpha && a->nu == b->nu && a->beta == b->beta;
+}
+
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
                     const lvc_problem *probs, int precision, int seg_x,
                     int seg_y) {
@@ -92,16 +109,21 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   if (n == 0)
     return 0;
   const size_t real = precision == LVC_FP64 ? sizeof(double) : sizeof(float);
+  const int xshare = lvc_xshare(ctx, precision);
   b->h_probs = malloc((size_t)n * sizeof(lvc_problem));
-  if (!b->h_probs) {
+  int *order = malloc((size_t)n * sizeof(int));
+  int *gsize = malloc((size_t)n * sizeof(int)); /* by position in order */
+  int32_t *leaders = malloc((size_t)n * sizeof(int32_t));
+  if (!b->h_probs || !order || !gsize || !leaders) {
+    free(order);
+    free(gsize);
+    free(leaders);
     lvc_set_error(ctx, "host allocation failed");
     return 1;
   }
   int64_t geo = 0, state = 0, nxt = 0, nyt = 0, steps = 0;
   for (int p = 0; p < n; p++) {
     lvc_problem q = probs[p];
-    const int gx = lvc_lanes_for(q.nx, seg_x);
-    const int rows = LVC_BLOCK / gx;
     int gy, cpl, cols;
     lvc_ytile_shape(q.ny, seg_y, (int)real, &gy, &cpl, &cols);
     const int lds = lvc_ytile_lds(q.ny, seg_y, (int)real);
@@ -112,16 +134,41 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     q.k = -1;
     q.status = 0;
     q.ydone = 0;
+    q.fac_src = p;
     q.ytiles = (q.nx + cols - 1) / cols;
     geo += align64(LVC_GEO_LEN(q.nx, q.ny));
     state += align64((int64_t)q.nx * q.ny);
-    nxt += (q.ny + rows - 1) / rows;
     nyt += q.ytiles;
     if (q.nt - 1 > steps)
       steps = q.nt - 1;
     b->h_probs[p] = q;
   }
+  /* Problems by numT, longest first (stable), so that at every step the
+   * tiles of the problems still running form a prefix of each list; runs of
+   * up to xshare consecutive problems with the same factorisation share
+   * their x tiles (and the first one's y factor table). */
+  for (int p = 0; p < n; p++)
+    order[p] = p;
+  nt_sort_desc(order, n, b->h_probs);
+  for (int oi = 0; oi < n;) {
+    const int p = order[oi];
+    int cnt = 1;
+    while (cnt < xshare && oi + cnt < n && order[oi + cnt] == p + cnt &&
+           same_factor(&b->h_probs[p], &b->h_probs[p + cnt]))
+      cnt++;
+    gsize[oi] = cnt;
+    if (cnt > 1)
+      b->xshared = 1;
+    for (int i = 0; i < cnt; i++)
+      b->h_probs[p + i].fac_src = p;
+    const int rows = LVC_BLOCK / lvc_lanes_for(b->h_probs[p].nx, seg_x);
+    nxt += (b->h_probs[p].ny + rows - 1) / rows;
+    oi += cnt;
+  }
   if (nxt + nyt > (int64_t)1 << 30) {
+    free(order);
+    free(gsize);
+    free(leaders);
     lvc_set_error(ctx, "batch too large");
     return 1;
   }
@@ -129,44 +176,43 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   b->nyt = (int)nyt;
   b->steps = steps;
   lvc_tile *tiles = malloc((size_t)(nxt + nyt) * sizeof(lvc_tile));
-  int *order = malloc((size_t)n * sizeof(int));
   b->x_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
   b->y_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
   b->f_active = malloc((size_t)(steps ? steps : 1) * sizeof(int));
-  if (!tiles || !order || !b->x_active || !b->y_active || !b->f_active) {
+  if (!tiles || !b->x_active || !b->y_active || !b->f_active) {
     free(tiles);
     free(order);
+    free(gsize);
+    free(leaders);
     lvc_set_error(ctx, "host allocation failed");
     return 1;
   }
-  /* Problems by numT, longest first (stable), so that at every step the
-   * tiles of the problems still running form a prefix of each list. */
-  for (int p = 0; p < n; p++)
-    order[p] = p;
-  nt_sort_desc(order, n, b->h_probs);
   int64_t tx = 0, ty = nxt;
+  int nlead = 0;
   for (int64_t s = 0; s < steps; s++)
     b->x_active[s] = b->y_active[s] = b->f_active[s] = 0;
-  for (int oi = 0; oi < n; oi++) {
-    const int p = order[oi];
+  for (int oi = 0; oi < n; oi += gsize[oi]) {
+    const int p = order[oi], cnt = gsize[oi];
     const lvc_problem *q = &b->h_probs[p];
     const int gx = lvc_lanes_for(q->nx, seg_x);
     const int rows = LVC_BLOCK / gx;
     int gy, cpl, cols;
     lvc_ytile_shape(q->ny, seg_y, (int)real, &gy, &cpl, &cols);
     for (int r0 = 0; r0 < q->ny; r0 += rows) {
-      lvc_tile t = {p, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx, 1, 0};
+      lvc_tile t = {p, r0, q->ny - r0 < rows ? q->ny - r0 : rows, gx, cnt, 0};
       tiles[tx++] = t;
     }
-    for (int c0 = 0; c0 < q->nx; c0 += cols) {
-      lvc_tile t = {p, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy, cpl, 0};
-      tiles[ty++] = t;
-    }
-    /* this problem runs steps 0 .. nt-2: the prefixes reach past its tiles */
+    for (int i = 0; i < cnt; i++)
+      for (int c0 = 0; c0 < q->nx; c0 += cols) {
+        lvc_tile t = {p + i, c0, q->nx - c0 < cols ? q->nx - c0 : cols, gy, cpl, 0};
+        tiles[ty++] = t;
+      }
+    leaders[nlead++] = p;
+    /* these problems run steps 0 .. nt-2: the prefixes reach past their tiles */
     for (int64_t s = 0; s < q->nt - 1; s++) {
       b->x_active[s] = (int)tx;
       b->y_active[s] = (int)(ty - nxt);
-      b->f_active[s] = oi + 1;
+      b->f_active[s] = nlead;
     }
   }
   b->probs_bytes = (size_t)n * sizeof(lvc_problem);
@@ -185,9 +231,11 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   if (!rc)
     rc = lvc_upload(ctx, b->d_probs, b->h_probs, b->probs_bytes) ||
          lvc_upload(ctx, b->d_tiles, tiles, b->tiles_bytes) ||
-         (b->d_fprobs && lvc_upload(ctx, b->d_fprobs, order, (size_t)n * sizeof(int32_t)));
+         (b->d_fprobs && lvc_upload(ctx, b->d_fprobs, leaders, (size_t)nlead * sizeof(int32_t)));
   free(tiles);
   free(order);
+  free(gsize);
+  free(leaders);
   b->graph = ctx->cfg && ctx->cfg->graph;
   return rc;
 }
@@ -227,7 +275,7 @@ static int run_graph(struct futhark_context *ctx, lvc_batch *b,
     e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
                           b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
                           lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
-                          b->f_active[s]);
+                          b->f_active[s], b->xshared);
     if (!e)
       e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
                             b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds,
@@ -265,7 +313,7 @@ int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
     e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
                           b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
                           lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
-                          b->f_active[s]);
+                          b->f_active[s], b->xshared);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "x sweep kernel"))
       return 1;

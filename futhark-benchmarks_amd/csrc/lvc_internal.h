@@ -41,7 +41,8 @@ typedef struct lvc_problem {
   int32_t status;          /* 0 = ok, 3 = price index out of range */
   int32_t ytiles;          /* y-sweep workgroups of this problem per step */
   int32_t ydone;           /* y-sweep workgroups finished in this step */
-  int32_t pad_[4];
+  int32_t fac_src;         /* problem whose y factor table this one uses */
+  int32_t pad_[3];
   int64_t state_off;       /* element offset of R and Y */
   int64_t geo_off;         /* element offset into the f64 geometry buffer */
   double s0, strike, t, alpha, nu, beta;
@@ -49,13 +50,16 @@ typedef struct lvc_problem {
 
 /* One workgroup's share of a sweep: `count` rows (x sweep) or columns (y
  * sweep) of problem `problem`, starting at `first`, with `lanes` lanes (a
- * power of two) per row/column. */
+ * power of two) per row/column (column y sweep: segments per column). */
 typedef struct lvc_tile {
   int32_t problem;
   int32_t first;
   int32_t count;
   int32_t lanes;
-  int32_t cpl; /* y sweep: columns per lane (they share the factorisation) */
+  /* x sweep: problems problem .. problem+cpl-1, identical but for the strike,
+   * sharing the row factorisations; lane-segment y sweep: columns per lane
+   * (sharing the factorisation); column y sweep: 64-column groups */
+  int32_t cpl;
   int32_t pad_;
 } lvc_tile;
 
@@ -81,11 +85,14 @@ int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
                      double *geo, void *R);
 /* x sweep; with yseg = a column segment length (32, 64) it also writes the
  * y factor table for the step of each of the nfac problems fprobs[] into
- * yfac (real elements, LVC_YFAC_LEN per problem, indexed by problem). */
+ * yfac (real elements, LVC_YFAC_LEN per problem, indexed by problem).
+ * xshared: some tiles cover several problems (tile.cpl > 1); otherwise the
+ * single-problem variant runs (fewer registers). */
 int lvc_launch_xsweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *R, void *Y,
-                      void *yfac, int yseg, const int32_t *fprobs, int nfac);
+                      void *yfac, int yseg, const int32_t *fprobs, int nfac,
+                      int xshared);
 /* y sweep; column segments (seg >= 32) read the factor tables in yfac. */
 int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,

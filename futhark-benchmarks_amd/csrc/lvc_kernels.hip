@@ -35,6 +35,11 @@
 // operand modifiers, no LDS round trip); wider systems use ds_bpermute.
 // Lanes past the end of the system hold identity rows.
 //
+// Rounding: the file is compiled without floating-point contraction and every
+// fused multiply-add is written as fma(), so each operation's rounding is
+// fixed by the source, the same in every kernel variant that inlines it
+// (single-problem and shared x tiles, the server) whatever the scheduling.
+//
 // Precision: T = float or double for the state and the sweeps.  REF adds one
 // iterative-refinement pass to each implicit solve: the residual of the FP32
 // solution is formed in FP64 (with the same taps and the same rounded time
@@ -326,7 +331,7 @@ struct SegSys {
 #pragma unroll
     for (int k = 1; k < M - 1; ++k) {
       m[k] = a[k] * rb[k - 1];
-      g = s[k] - m[k] * g;
+      g = fma(-m[k], g, s[k]);
       rb[k] = rcp(g - c[k]);
     }
     // Homogeneous responses v (to x[first-1]) and w (to x[last]).
@@ -337,14 +342,14 @@ struct SegSys {
     w[M - 2] = -c[M - 2] * rb[M - 2];
 #pragma unroll
     for (int k = M - 3; k >= 0; --k) {
-      v[k] = (v[k] - c[k] * v[k + 1]) * rb[k];
+      v[k] = fma(-c[k], v[k + 1], v[k]) * rb[k];
       w[k] = -c[k] * w[k + 1] * rb[k];
     }
     // Reduced row of this lane: A X[seg-1] + B X[seg] + C X[seg+1] = D.
     const T vn = from_above(L, v[0], 1, T(0));
     const T wn = from_above(L, w[0], 1, T(0));
     T A = aL * v[M - 2];
-    T B = s[M - 1] - aL * (T(1) - w[M - 2]) - cL * (T(1) - vn);
+    T B = fma(-cL, T(1) - vn, fma(-aL, T(1) - w[M - 2], s[M - 1]));
     T C = cL * wn;
     // PCR on the matrix; the multipliers are kept for the right-hand sides.
 #pragma unroll
@@ -360,7 +365,7 @@ struct SegSys {
         k2[l] = q2;
         A = -Am * q1;
         C = -Cp * q2;
-        B = B - Cm * q1 - Ap * q2;
+        B = fma(-Ap, q2, fma(-Cm, q1, B));
       }
     }
     rB = rcp(B);
@@ -370,24 +375,24 @@ struct SegSys {
   __device__ __forceinline__ void solve(const T (&d)[M], T (&x)[M]) const {
     x[0] = d[0];
 #pragma unroll
-    for (int k = 1; k < M - 1; ++k) x[k] = d[k] - m[k] * x[k - 1];
+    for (int k = 1; k < M - 1; ++k) x[k] = fma(-m[k], x[k - 1], d[k]);
     x[M - 2] = x[M - 2] * rb[M - 2];
 #pragma unroll
-    for (int k = M - 3; k >= 0; --k) x[k] = (x[k] - c[k] * x[k + 1]) * rb[k];
+    for (int k = M - 3; k >= 0; --k) x[k] = fma(-c[k], x[k + 1], x[k]) * rb[k];
     const T un = from_above(L, x[0], 1, T(0));
-    T D = d[M - 1] - aL * x[M - 2] - cL * un;
+    T D = fma(-cL, un, fma(-aL, x[M - 2], d[M - 1]));
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
       const int st = 1 << l;
       if (st < L.G) {
         const T Dm = from_below(L, D, st, T(0)), Dp = from_above(L, D, st, T(0));
-        D = D - Dm * k1[l] - Dp * k2[l];
+        D = fma(-Dp, k2[l], fma(-Dm, k1[l], D));
       }
     }
     const T X = D * rB;
     const T XL = from_below(L, X, 1, T(0));
 #pragma unroll
-    for (int k = 0; k < M - 1; ++k) x[k] = x[k] + v[k] * XL + w[k] * X;
+    for (int k = 0; k < M - 1; ++k) x[k] = fma(w[k], X, fma(v[k], XL, x[k]));
     x[M - 1] = X;
   }
 };
@@ -407,7 +412,7 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
     const double xk = (double)x[k];
     const double xm = (double)(k > 0 ? x[k - 1] : xl);
     const double xp = (double)(k < M - 1 ? x[k + 1] : xr);
-    const double ax = (double)s[k] * xk + (double)a[k] * (xm - xk) + (double)c[k] * (xp - xk);
+    const double ax = fma((double)c[k], xp - xk, fma((double)a[k], xm - xk, (double)s[k] * xk));
     r[k] = (T)((double)d[k] - ax);
   }
   T e[M];
@@ -420,23 +425,30 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
 }
 
 // ---------------------------------------------------------------------------
-// x sweep: one workgroup = `count` rows of one problem, `lanes` lanes per row.
-// U (the right-hand side) and V (the explicit y term) wait in LDS while the
-// solver's registers are live.
+// x sweep: one workgroup = `count` rows of `cpl` consecutive problems that
+// share their factorisation (one problem unless they differ only in the
+// strike), `lanes` lanes per row.  U (the right-hand side) and V
+// (the explicit y term) wait in LDS while the solver's registers are live.
 // ---------------------------------------------------------------------------
 template <typename T, int M>
 constexpr int kXsweepLds = 2 * M * LVC_BLOCK;  // elements of T
 
-template <typename T, int M, bool REF>
+// NB = 1: single-problem tiles, the group loop resolved at compile time (fewer
+// registers); NB = 0: tile.cpl problems.  Both run the same arithmetic, so a
+// problem's result does not depend on which variant its batch used.
+template <typename T, int M, bool REF, int NB>
 __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
                                             const lvc_problem *__restrict__ probs,
                                             const double *__restrict__ geo,
                                             const T *__restrict__ R,
                                             T *__restrict__ Y, T *smem) {
   T *lds_u = smem, *lds_v = smem + M * LVC_BLOCK;
+  // The tile's problems tile.problem .. + tile.cpl - 1 differ only in their
+  // state (the strike): they share sizes, grid, taps, time step and so the
+  // factorisation of every row, which is computed once from the first.
   const lvc_problem &pr = probs[tile.problem];
   const int kt = pr.k;
-  if (kt < 0) return;  // problem finished: whole workgroup exits
+  if (kt < 0) return;  // problems finished: whole workgroup exits
   const int G = tile.lanes;
   const int seg = threadIdx.x & (G - 1);
   const int rloc = threadIdx.x / G;
@@ -449,55 +461,80 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
   const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
   const T dt = (T)(1.0 / (tnext - tnow));
   const double nu = pr.nu;
-  const T E = (T)exp(2.0 * g[3 * nx + j] - nu * nu * tnow);  // exp(2y - nu^2 t)
+  const T E = (T)exp(fma(-nu * nu, tnow, 2.0 * g[3 * nx + j]));  // exp(2y - nu^2 t)
   const T ayj = tp.ay[j], cyj = tp.cy[j];
   const bool vec = (nx & 3) == 0;
-
-  const T *row = R + pr.state_off + (int64_t)j * nx;
   const int i0 = seg * M;
-  T rc[M];
-  load_seg<T, M>(row, i0, nx, vec, rc);
-  const T rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
-  const T rr = (i0 + M < nx) ? row[i0 + M] : T(0);
-  T a[M], c[M], s[M], U[M];
-  {
-    T rm[M], rp[M];
+
+  // Rows j-1, j, j+1 of problem b.  A single problem's rows and explicit
+  // terms come before the factorisation (the loads overlap it).  In a group,
+  // without refinement each next problem's rows are loaded while the current
+  // one is solved; with it, the registers that would take are worth more as
+  // occupancy (measured), so they load at the top of each iteration.
+  constexpr bool kPrefetch = !REF;
+  T rc[M], rm[M], rp[M], rl, rr;
+  auto load_rows = [&](int b) {
+    const T *row = R + probs[tile.problem + b].state_off + (int64_t)j * nx;
+    load_seg<T, M>(row, i0, nx, vec, rc);
+    rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
+    rr = (i0 + M < nx) ? row[i0 + M] : T(0);
     if (j > 0) load_seg<T, M>(row - nx, i0, nx, vec, rm);
     if (j < ny - 1) load_seg<T, M>(row + nx, i0, nx, vec, rp);
-    load_seg<T, M>(tp.ax, i0, nx, vec, a);
-    load_seg<T, M>(tp.cx, i0, nx, vec, c);
+  };
+  T a[M], c[M], s[M];
+  // U (the right-hand side) to registers; V (the explicit y term) and, for
+  // the refinement, U to LDS.
+  auto explicit_terms = [&](T (&U)[M]) {
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-      const bool in = i0 + k < nx;
-      a[k] *= E;
-      c[k] *= E;
-      s[k] = in ? dt : T(1);
       const T r0 = rc[k];
       const T rw = k > 0 ? rc[k - 1] : rl;
       const T re = k < M - 1 ? rc[k + 1] : rr;
       const T dm = j > 0 ? rm[k] - r0 : T(0);
       const T dp = j < ny - 1 ? rp[k] - r0 : T(0);
-      const T ux = a[k] * (rw - r0) + c[k] * (re - r0);  // 0.5 * explicitX
-      const T vy = T(-2) * (ayj * dm + cyj * dp);          // explicitY
+      const T ux = fma(c[k], re - r0, a[k] * (rw - r0));  // 0.5 * explicitX
+      const T vy = T(-2) * fma(cyj, dp, ayj * dm);         // explicitY
       lds_v[k * LVC_BLOCK + threadIdx.x] = vy;
-      U[k] = dt * r0 - ux + vy;
+      U[k] = fma(dt, r0, -ux) + vy;
       if (REF) lds_u[k * LVC_BLOCK + threadIdx.x] = U[k];
     }
+  };
+  if (NB == 1 || kPrefetch) load_rows(0);
+  load_seg<T, M>(tp.ax, i0, nx, vec, a);
+  load_seg<T, M>(tp.cx, i0, nx, vec, c);
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    a[k] *= E;
+    c[k] *= E;
+    s[k] = i0 + k < nx ? dt : T(1);
   }
+  T U[M];
+  if (NB == 1) explicit_terms(U);
   SegSys<T, M> sys;
   sys.prepare(a, c, s, make_lanes(seg, G));
-  T x[M];
-  sys.solve(U, x);
-  if (REF) {
-    T d[M];
+
+  const int nb = NB > 0 ? NB : tile.cpl;
+  for (int b = 0; b < nb; ++b) {
+    const int64_t off = probs[tile.problem + b].state_off + (int64_t)j * nx;
+    if (NB != 1) {
+      if (!kPrefetch) load_rows(b);
+      explicit_terms(U);
+      if (kPrefetch && b + 1 < nb) load_rows(b + 1);
+    }
+    T x[M];
+    sys.solve(U, x);
+    if (REF) {
+      T d[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) d[k] = lds_u[k * LVC_BLOCK + threadIdx.x];
-    refine(sys, d, a, c, s, x);
+      for (int k = 0; k < M; ++k) d[k] = lds_u[k * LVC_BLOCK + threadIdx.x];
+      refine(sys, d, a, c, s, x);
+    }
+    T out[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      out[k] = fma(dt, x[k], T(-0.5) * lds_v[k * LVC_BLOCK + threadIdx.x]);
+    store_seg<T, M>(Y + off, i0, nx, vec, out);
   }
-  T out[M];
-#pragma unroll
-  for (int k = 0; k < M; ++k) out[k] = dt * x[k] - T(0.5) * lds_v[k * LVC_BLOCK + threadIdx.x];
-  store_seg<T, M>(Y + pr.state_off + (int64_t)j * nx, i0, nx, vec, out);
 }
 
 constexpr int kFacScratch = 6 * LVC_COL_ROWS;  // elements of the factor scratch
@@ -509,8 +546,14 @@ __device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
 // nfac workgroups of the grid each write one problem's y factor for this step
 // (problem fprobs[blockIdx.x], wave 0 at raised priority: a serial chain that
 // should finish long before the row tiles do); the row tiles follow.
-template <typename T, int M, bool REF>
-__global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
+// Occupancy floor (waves per SIMD) for the shared FP32+refinement tile with
+// 8-unknown segments, which the compiler would otherwise give 198 registers
+// (2 waves); 3 waves fit without spills.
+template <typename T, int M, bool REF, bool SHARE>
+constexpr int kXsweepWaves = (sizeof(T) == 4 && M == 8 && SHARE && REF) ? 3 : 1;
+
+template <typename T, int M, bool REF, bool SHARE>
+__global__ __launch_bounds__(LVC_BLOCK, (kXsweepWaves<T, M, REF, SHARE>)) void lvc_xsweep_kernel(
     const lvc_problem *__restrict__ probs, const lvc_tile *__restrict__ tiles,
     const double *__restrict__ geo, const T *__restrict__ R, T *__restrict__ Y,
     T *__restrict__ yfac, int yseg, const int32_t *__restrict__ fprobs, int nfac) {
@@ -531,7 +574,7 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_xsweep_kernel(
     }
     return;
   }
-  xsweep_tile<T, M, REF>(tiles[blockIdx.x - nfac], probs, geo, R, Y, smem);
+  xsweep_tile<T, M, REF, SHARE ? 0 : 1>(tiles[blockIdx.x - nfac], probs, geo, R, Y, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -713,10 +756,10 @@ __device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
       a = A(r);
       c = C(r);
       const T m = a * rb;
-      gg = S(r) - m * gg;
+      gg = fma(-m, gg, S(r));
       rb = rcp(gg - c);
       v = -m * v;
-      z = S(r) - m * z;
+      z = fma(-m, z, S(r));
       fm[r] = m;
       frb[r] = srb[r] = rb;
       fcrb[r] = scrb[r] = c * rb;
@@ -735,8 +778,8 @@ __device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
     for (int k = M - 3; k >= 0; --k) {
       const int r = r0 + k;
       const T crb = scrb[r], rbk = srb[r];
-      v = sv[r] * rbk - crb * v;
-      z = sz[r] * rbk - crb * z;
+      v = fma(-crb, v, sv[r] * rbk);
+      z = fma(-crb, z, sz[r] * rbk);
       w = -crb * w;
       fv[r] = v;
       fw[r] = w;
@@ -757,12 +800,12 @@ __device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
   const T w0n = __shfl(w0, q + 1), z0n = __shfl(z0, q + 1);
   const bool last = q + 1 >= P;
   const T Aq = aL * vL, Cq = last ? T(0) : cL * w0n;
-  const T delta = sL - aL * zL - (last ? T(0) : cL * z0n);
+  const T delta = fma(-cL, last ? T(0) : z0n, fma(-aL, zL, sL));
   T gq = T(0), rrb = T(0);
   for (int j = 0; j < P; ++j) {  // serial over the segments, in every lane
     const T Aj = __shfl(Aq, j), Cj = __shfl(Cq, j), dj = __shfl(delta, j);
     const T m = j == 0 ? T(0) : Aj * rrb;
-    gq = j == 0 ? dj - Aj : dj - m * gq;
+    gq = j == 0 ? dj - Aj : fma(-m, gq, dj);
     rrb = rcp(gq - Cj);
     if (q == 0) {
       fac_seg(f, kSrm)[j] = m;
@@ -791,13 +834,14 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
   const int r0 = p * M;
   const T dlast = x[M - 1];
 #pragma unroll
-  for (int k = 1; k < M - 1; ++k) x[k] -= L.m[r0 + k] * x[k - 1];
+  for (int k = 1; k < M - 1; ++k) x[k] = fma(-L.m[r0 + k], x[k - 1], x[k]);
   x[M - 2] *= L.rb[r0 + M - 2];
 #pragma unroll
-  for (int k = M - 3; k >= 0; --k) x[k] = x[k] * L.rb[r0 + k] - L.crb[r0 + k] * x[k + 1];
+  for (int k = M - 3; k >= 0; --k)
+    x[k] = fma(-L.crb[r0 + k], x[k + 1], x[k] * L.rb[r0 + k]);
   const int me = (gbase + p) * 64 + lane;
   L.y0[sl][me] = x[0];
-  L.Dt[sl][me] = dlast - L.aL[p] * x[M - 2];
+  L.Dt[sl][me] = fma(-L.aL[p], x[M - 2], dlast);
   __syncthreads();
   // Reduced system of the column: Thomas over its P segments (slots
   // gbase .. gbase+P-1, all below kColWaves).
@@ -809,8 +853,8 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
       const int at = (gbase + q) * 64 + lane;
       const T yn = q + 1 < P ? L.y0[sl][at + 64] : T(0);
       if (q == p) y0n = yn;
-      const T D = L.Dt[sl][at] - L.cL[q] * yn;
-      prev = q == 0 ? D : D - L.rm[q] * prev;
+      const T D = fma(-L.cL[q], yn, L.Dt[sl][at]);
+      prev = q == 0 ? D : fma(-L.rm[q], prev, D);
       Yr[q] = prev;
     }
   }
@@ -818,7 +862,7 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
 #pragma unroll
   for (int q = kColWaves - 1; q >= 0; --q) {
     if (q < P) {
-      nxt = (Yr[q] - L.rC[q] * nxt) * L.rrb[q];
+      nxt = fma(-L.rC[q], nxt, Yr[q]) * L.rrb[q];
       if (q == p + 1) Xp = nxt;
       if (q == p) Xc = nxt;
       if (q == p - 1) Xm = nxt;
@@ -883,7 +927,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   }
 
   // The step's factor table (written by the x sweep) into LDS.
-  const T *f = yfac + (int64_t)tile.problem * LVC_YFAC_LEN;
+  const T *f = yfac + (int64_t)pr.fac_src * LVC_YFAC_LEN;
   for (int r = threadIdx.x; r < P * M; r += nact) {
     L.m[r] = f[kFm * LVC_COL_ROWS + r];
     L.rb[r] = f[kFrb * LVC_COL_ROWS + r];
@@ -928,7 +972,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
       const double xm = (double)(k > 0 ? x[k - 1] : XL);
       const double xp = (double)(k < M - 1 ? x[k + 1] : XN);
       const double sk = r0 + k < ny ? s64 : 1.0;
-      const double ax = sk * xk + L.a64[r0 + k] * (xm - xk) + L.c64[r0 + k] * (xp - xk);
+      const double ax = fma(L.c64[r0 + k], xp - xk, fma(L.a64[r0 + k], xm - xk, sk * xk));
       d[k] = (T)((double)d[k] - ax);
     }
     T EL, EN;
@@ -976,7 +1020,7 @@ template <typename T, int M, bool REF>
 int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
                   const lvc_tile *tiles, int ntiles, const double *geo,
                   const void *in, void *out, int lds_bytes, void *yfac, int yseg,
-                  const int32_t *fprobs, int nfac) {
+                  const int32_t *fprobs, int nfac, bool xshared) {
   if (ntiles <= 0) return 0;
   if constexpr (M >= 32) {
     if (xs || !yfac) return (int)hipErrorInvalidValue;  // column segments: y sweep only
@@ -988,8 +1032,12 @@ int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
     if (xs) {
       if (yseg > 0 && (!yfac || !fprobs || !lvc_seg_is_col(yseg))) return (int)hipErrorInvalidValue;
       if (yseg <= 0 || nfac < 0) nfac = 0;
-      lvc_xsweep_kernel<T, M, REF><<<ntiles + nfac, LVC_BLOCK, 0, st>>>(
-          probs, tiles, geo, (const T *)in, (T *)out, (T *)yfac, yseg, fprobs, nfac);
+      if (xshared)
+        lvc_xsweep_kernel<T, M, REF, true><<<ntiles + nfac, LVC_BLOCK, 0, st>>>(
+            probs, tiles, geo, (const T *)in, (T *)out, (T *)yfac, yseg, fprobs, nfac);
+      else
+        lvc_xsweep_kernel<T, M, REF, false><<<ntiles + nfac, LVC_BLOCK, 0, st>>>(
+            probs, tiles, geo, (const T *)in, (T *)out, (T *)yfac, yseg, fprobs, nfac);
     } else {
       if (lds_bytes <= 0 || lds_bytes > 65536) return (int)hipErrorInvalidValue;
       lvc_ysweep_kernel<T, M, REF><<<ntiles, LVC_BLOCK, lds_bytes, st>>>(
@@ -1003,11 +1051,11 @@ template <typename T, bool REF>
 int dispatch_seg(hipStream_t st, bool xs, int seg, const lvc_problem *probs,
                  const lvc_tile *tiles, int ntiles, const double *geo,
                  const void *in, void *out, int lds, void *yfac, int yseg,
-                 const int32_t *fprobs, int nfac) {
+                 const int32_t *fprobs, int nfac, bool xshared) {
 #define LVC_SEG_CASE(S)                                                         \
   case S:                                                                      \
     return launch_sweeps<T, S, REF>(st, xs, probs, tiles, ntiles, geo, in, out, \
-                                    lds, yfac, yseg, fprobs, nfac);
+                                    lds, yfac, yseg, fprobs, nfac, xshared);
   switch (seg) {
     LVC_SEG_CASE(4)
     LVC_SEG_CASE(8)
@@ -1022,17 +1070,17 @@ int dispatch_seg(hipStream_t st, bool xs, int seg, const lvc_problem *probs,
 int dispatch(hipStream_t st, bool xs, int precision, int seg,
              const lvc_problem *probs, const lvc_tile *tiles, int ntiles,
              const double *geo, const void *in, void *out, int lds, void *yfac,
-             int yseg, const int32_t *fprobs, int nfac) {
+             int yseg, const int32_t *fprobs, int nfac, bool xshared) {
   switch (precision) {
     case LVC_FP32:
       return dispatch_seg<float, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out,
-                                        lds, yfac, yseg, fprobs, nfac);
+                                        lds, yfac, yseg, fprobs, nfac, xshared);
     case LVC_FP32_REFINE:
       return dispatch_seg<float, true>(st, xs, seg, probs, tiles, ntiles, geo, in, out,
-                                       lds, yfac, yseg, fprobs, nfac);
+                                       lds, yfac, yseg, fprobs, nfac, xshared);
     case LVC_FP64:
       return dispatch_seg<double, false>(st, xs, seg, probs, tiles, ntiles, geo, in, out,
-                                         lds, yfac, yseg, fprobs, nfac);
+                                         lds, yfac, yseg, fprobs, nfac, xshared);
     default: return (int)hipErrorInvalidValue;
   }
 }
@@ -1060,9 +1108,10 @@ int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
 int lvc_launch_xsweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *R, void *Y,
-                      void *yfac, int yseg, const int32_t *fprobs, int nfac) {
+                      void *yfac, int yseg, const int32_t *fprobs, int nfac,
+                      int xshared) {
   return dispatch((hipStream_t)stream, true, precision, seg, probs, tiles,
-                  ntiles, geo, R, Y, 0, yfac, yseg, fprobs, nfac);
+                  ntiles, geo, R, Y, 0, yfac, yseg, fprobs, nfac, xshared != 0);
 }
 
 int lvc_launch_ysweep(void *stream, int precision, int seg,
@@ -1070,7 +1119,8 @@ int lvc_launch_ysweep(void *stream, int precision, int seg,
                       int ntiles, const double *geo, const void *Y, void *R,
                       int lds_bytes, const void *yfac) {
   return dispatch((hipStream_t)stream, false, precision, seg, probs, tiles,
-                  ntiles, geo, Y, R, lds_bytes, const_cast<void *>(yfac), 0, nullptr, 0);
+                  ntiles, geo, Y, R, lds_bytes, const_cast<void *>(yfac), 0, nullptr, 0,
+                  false);
 }
 
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,

@@ -51,9 +51,10 @@ int lvc_ready(struct futhark_context *ctx) {
 
 static const char *const tuning_names[] = {
     "main.suff_intra_par_0", "main.suff_intra_par_1", "lvc.precision",
-    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.graph"};
+    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.graph", "lvc.xshare"};
 static const char *const tuning_classes[] = {
-    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag"};
+    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag",
+    "count"};
 #define NTUNING ((int)(sizeof(tuning_names) / sizeof(tuning_names[0])))
 
 int futhark_get_tuning_param_count(void) { return NTUNING; }
@@ -71,6 +72,7 @@ struct futhark_context_config *futhark_context_config_new(void) {
   cfg->device = 0;
   cfg->precision = LVC_FP32_REFINE;
   cfg->graph = 0;
+  cfg->xshare = 0;
   cfg->suff_intra_par_0 = 32; /* finpar/LocVolCalib.fut.tuning:1-2 */
   cfg->suff_intra_par_1 = 32;
   return cfg;
@@ -128,6 +130,12 @@ int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
       cfg->seg_x = (int)v;
     else
       cfg->seg_y = (int)v;
+    return 0;
+  }
+  if (strcmp(name, "lvc.xshare") == 0) {
+    if (v > LVC_XSHARE_MAX)
+      return 1;
+    cfg->xshare = (int)v;
     return 0;
   }
   if (strcmp(name, "lvc.graph") == 0) {

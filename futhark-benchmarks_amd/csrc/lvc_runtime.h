@@ -13,6 +13,7 @@
 #include "lvc_internal.h"
 
 #define LVC_MAX_TUNED 64
+#define LVC_XSHARE_MAX 16
 #define LVC_MAX_PROF 16
 
 struct futhark_context_config {
@@ -23,6 +24,8 @@ struct futhark_context_config {
   int seg_x, seg_y; /* 0 = tuned */
   int autotune;
   int graph; /* replay the step launches from a HIP graph */
+  int xshare; /* problems per x tile sharing a factorisation, 1..LVC_XSHARE_MAX;
+                0 = the default of the precision */
   size_t suff_intra_par_0, suff_intra_par_1;
 };
 
@@ -124,6 +127,7 @@ typedef struct lvc_batch {
   void *d_yfac; /* y factor tables (column y sweep only), LVC_YFAC_LEN each */
   size_t yfac_bytes;
   int32_t *d_fprobs; /* problems by numT, longest first (factor workgroups) */
+  int xshared;       /* some x tiles cover several problems */
   int *f_active;     /* [steps] problems still running at step s */
 } lvc_batch;
 
@@ -138,5 +142,6 @@ int lvc_validate_sizes(struct futhark_context *ctx, int64_t nx, int64_t ny,
                        int64_t nt);
 void lvc_choose_segs(struct futhark_context *ctx, int precision, int nx, int ny,
                      int *seg_x, int *seg_y);
+int lvc_xshare(struct futhark_context *ctx, int precision);
 
 #endif /* LVC_RUNTIME_H */

@@ -295,6 +295,7 @@ static int admit(struct lvc_server *s) {
     q.k = -1;
     q.state_off = (int64_t)slot * s->slot_elems;
     q.geo_off = (int64_t)slot * s->slot_geo;
+    q.fac_src = slot;
     {
       int gy, cpl, cols;
       lvc_ytile_shape(q.ny, s->seg_y, (int)s->real, &gy, &cpl, &cols);
@@ -333,7 +334,7 @@ int32_t lvc_server_step(struct lvc_server *s) {
     int e = lvc_launch_xsweep(st, s->precision, s->seg_x, s->d_slots, s->d_tiles,
                               s->nxt, s->d_geo, s->d_R, s->d_Y, s->d_yfac,
                               lvc_seg_is_col(s->seg_y) ? s->seg_y : 0, s->d_fprobs,
-                              s->nfac);
+                              s->nfac, 0);
     lvc_prof_end(ctx, tok);
     rc = lvc_check_hip(ctx, e, "x sweep kernel");
     if (!rc) {

@@ -59,6 +59,9 @@ def test_config_and_tuning_params():
     for seg in (4, 8, 16, 32, 64):
         assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_y", seg) == 0
     assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.seg_y", 128) != 0
+    for share in (1, 4, 16):
+        assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.xshare", share) == 0
+    assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.xshare", 17) != 0
     L.futhark_context_config_set_device(cfg, b"hip:0")
     L.futhark_context_config_set_cache_file(cfg, b"/tmp/lvc-tune-cache.txt")
     L.futhark_context_config_free(cfg)

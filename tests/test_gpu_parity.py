@@ -255,6 +255,35 @@ def test_y_segments_vs_oracle(seg_y, shapes, precision):
         assert err < F32R_REL
 
 
+def test_factor_sharing_bit_identical():
+    """x tiles shared by problems that differ only in the strike (the
+    reference's main, and runs of such problems in a batch) give the same bits
+    as one problem per tile."""
+    say("case: x-factor sharing (lvc.xshare 1 / 4 / 16), main and a grouped batch")
+    rng = np.random.default_rng(5)
+    n = 20
+    base = dict(numX=np.full(n, 64, np.int32), numY=np.full(n, 96, np.int32),
+                numT=np.full(n, 7, np.int32), s0=np.full(n, 0.03), t=np.full(n, 4.0),
+                alpha=np.full(n, 0.2), nu=np.full(n, 0.6), beta=np.full(n, 0.5),
+                strike=rng.uniform(0.0, 0.05, n))
+    # break the run in places: other parameters, other sizes, other numT
+    base["nu"][5] = 0.55
+    base["numX"][11] = 32
+    base["numT"][15:] = 5
+    out = {}
+    for xs in (1, 4, 16):
+        with lvc.LocVolCalib(tuning={"lvc.xshare": xs}) as ctx:
+            out[xs] = [ctx.main(24, 64, 64, 9, *CFG.values()),
+                       ctx.main_f64(24, 64, 64, 9, *CFG.values()),
+                       ctx.main_batch(**base, precision=lvc.PRECISION_FP32_REFINE),
+                       ctx.main_batch(**base, precision=lvc.PRECISION_FP64)]
+    for xs in (4, 16):
+        for a, b in zip(out[1], out[xs]):
+            assert np.array_equal(a, b)
+    ref = oracle_batch(base)
+    assert rel_err(out[4][3], ref) < F64_REL
+
+
 def test_empty_and_errors(gpu_ctx):
     say("case: outer=0 and invalid sizes")
     assert gpu_ctx.main_f64(0, 32, 32, 8, *CFG.values()).shape == (0,)
