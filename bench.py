@@ -71,26 +71,49 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v) * 1e3, q))
 
 
-def cpu_baseline(budget_s=12.0):
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=12.0, single_budget_s=4.0):
+    """The oracle (f32, the reference's scan-form tridagPar) on the host:
+    OpenMP over options on all the cores this process may use (the analogue
+    of futhark multicore), then one thread (futhark c), each on a
+    time-bounded sample of the headline workload's options."""
     import numpy as np
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
-    chunk = 2 * threads
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        idx = np.arange(done, done + chunk) % OUTER
-        K = (np.float32(0.001) * idx.astype(np.float32)).astype(np.float32)
-        n = len(idx)
-        oracle.batch([NX] * n, [NY] * n, [NT] * n, [PARAMS[0]] * n, K, [PARAMS[1]] * n,
-                     [PARAMS[2]] * n, [PARAMS[3]] * n, [PARAMS[4]] * n,
-                     dtype=np.float32, thomas=False, nthreads=threads)
-        done += n
-    el = time.perf_counter() - t0
+
+    def sample(nthreads, budget):
+        chunk = 2 * nthreads
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            idx = np.arange(done, done + chunk) % OUTER
+            K = (np.float32(0.001) * idx.astype(np.float32)).astype(np.float32)
+            n = len(idx)
+            oracle.batch([NX] * n, [NY] * n, [NT] * n, [PARAMS[0]] * n, K, [PARAMS[1]] * n,
+                         [PARAMS[2]] * n, [PARAMS[3]] * n, [PARAMS[4]] * n,
+                         dtype=np.float32, thomas=False, nthreads=nthreads)
+            done += n
+        return done, time.perf_counter() - t0
+
+    done, el = sample(threads, budget_s)
+    done1, el1 = sample(1, single_budget_s)
     return {"value": round(done / el, 3), "unit": "options/s", "cores": threads,
             "kind": "port",
             "sample": f"{done} options of main({OUTER},{NX},{NY},{NT}) (f32, scan-form "
-                      f"tridagPar as LocVolCalib.fut:68-124), OpenMP over options, {el:.1f} s"}
+                      f"tridagPar as LocVolCalib.fut:68-124), OpenMP over options, {el:.1f} s",
+            "single_thread": {"value": round(done1 / el1, 3), "unit": "options/s",
+                              "sample": f"{done1} options, 1 thread, {el1:.1f} s"},
+            "host": host_cpu_model()}
 
 
 def uneven_problems(n, seed, sizes=(32, 64, 128, 256), nt=(32, 256)):

@@ -264,6 +264,13 @@ class LocVolCalib:
     def free_f64(self, h: int) -> None:
         self._free("f64", h)
 
+    def values_f32(self, h: int) -> np.ndarray:
+        """Copy of a device f32 array handle's values (the handle stays)."""
+        return self._values("f32", h)
+
+    def values_f64(self, h: int) -> np.ndarray:
+        return self._values("f64", h)
+
     def main(self, outer, numX, numY, numT, s0, t, alpha, nu, beta) -> np.ndarray:
         """LocVolCalib.fut:238-252: ``outer`` prices (float32)."""
         h = self.main_raw(outer, numX, numY, numT, s0, t, alpha, nu, beta)

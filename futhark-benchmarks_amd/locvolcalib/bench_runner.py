@@ -1,0 +1,178 @@
+"""A ``futhark bench``-shaped runner for the LocVolCalib entry point.
+
+Reads the datasets of the reference's benchmark stanza (LocVolCalib.fut:1-10:
+``compiled input @ LocVolCalib-data/small.in`` / ``output @ ...small.out``)
+in Futhark's text or binary value format, runs ``main`` on each one a number
+of times through the C ABI, checks the result against the expected output,
+and reports per-run wall times in microseconds -- on stdout as one line per
+dataset, and with ``--json FILE`` in the JSON layout ``futhark bench
+--json`` writes (README.md:36-37)::
+
+    {"finpar/LocVolCalib.fut:main":
+       {"datasets": {"LocVolCalib-data/small.in": {"runtimes": [...],
+                                                    "stderr": ""}}}}
+
+A run is the entry call plus the context sync that ends it (inputs are
+scalars, the result stays on the device); one warm-up run precedes the timed
+ones, as ``futhark bench`` does.
+
+Usage::
+
+    python -m locvolcalib.bench_runner --program path/to/finpar/LocVolCalib.fut
+    python -m locvolcalib.bench_runner path/to/small.in path/to/medium.in ...
+
+The expected output of ``X.in`` is ``X.out`` beside it (or the stanza's
+``output @`` file).  Results are compared element-wise with an absolute
+tolerance (default 5e-6: the reference's f32 outputs are printed to 6
+significant digits and carry their own rounding).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import futhark_values
+
+PROGRAM_KEY = "finpar/LocVolCalib.fut:main"
+
+
+def parse_stanza(text: str) -> List[Tuple[str, Optional[str]]]:
+    """(input, output) dataset paths of a Futhark test stanza: the comment
+    block that starts at ``-- ==`` (LocVolCalib.fut:2-10)."""
+    lines = text.splitlines()
+    try:
+        start = next(i for i, ln in enumerate(lines) if ln.strip().startswith("-- =="))
+    except StopIteration:
+        return []
+    pairs: List[Tuple[str, Optional[str]]] = []
+    for ln in lines[start + 1:]:
+        s = ln.strip()
+        if not s.startswith("--"):
+            break
+        body = s[2:].strip()
+        m = re.match(r"(?:compiled\s+)?input\s*@\s*(\S+)", body)
+        if m:
+            pairs.append((m.group(1), None))
+            continue
+        m = re.match(r"output\s*@\s*(\S+)", body)
+        if m and pairs and pairs[-1][1] is None:
+            pairs[-1] = (pairs[-1][0], m.group(1))
+    return pairs
+
+
+def main_arguments(values) -> tuple:
+    """The nine arguments of ``main`` (LocVolCalib.fut:238-246) from a
+    dataset's values: outer, numX, numY, numT (i32) and s0, t, alpha, nu,
+    beta (f32)."""
+    if len(values) != 9:
+        raise ValueError(f"main takes 9 values, the dataset has {len(values)}")
+    ints = tuple(int(v) for v in values[:4])
+    reals = tuple(float(np.float32(v)) for v in values[4:])
+    return ints + reals
+
+
+def compare(got: np.ndarray, expected: np.ndarray, abs_tol: float) -> Tuple[bool, float]:
+    got = np.asarray(got, np.float64)
+    expected = np.asarray(expected, np.float64)
+    if got.shape != expected.shape:
+        return False, float("inf")
+    if got.size == 0:
+        return True, 0.0
+    err = float(np.max(np.abs(got - expected)))
+    return err <= abs_tol, err
+
+
+def run_dataset(ctx, args: tuple, precision: int, runs: int) -> Tuple[List[int], np.ndarray]:
+    """Timed runs of main on one dataset: (runtimes in us, the result)."""
+    if precision == 2:
+        raw, free, fetch = ctx.main_f64_raw, ctx.free_f64, ctx.values_f64
+    else:
+        raw, free, fetch = ctx.main_raw, ctx.free_f32, ctx.values_f32
+    free(raw(*args))  # warm-up
+    ctx.sync()
+    times = []
+    result = None
+    for r in range(runs):
+        t0 = time.perf_counter()
+        h = raw(*args)
+        ctx.sync()
+        times.append(int(round((time.perf_counter() - t0) * 1e6)))
+        if r == runs - 1:
+            result = fetch(h)
+        free(h)
+    return times, result
+
+
+def bench(datasets: List[Tuple[str, Optional[str]]], runs: int = 10, precision: int = 1,
+          abs_tol: float = 5e-6, device: int = 0, names: Optional[List[str]] = None,
+          out=sys.stdout) -> Tuple[Dict, bool]:
+    """Runs every dataset; returns (the futhark-bench JSON object, all valid)."""
+    from . import LocVolCalib
+
+    results: Dict[str, Dict] = {}
+    all_ok = True
+    with LocVolCalib(device=device, precision=precision) as ctx:
+        for i, (inp, outp) in enumerate(datasets):
+            name = names[i] if names else inp
+            args = main_arguments(futhark_values.load(inp))
+            times, got = run_dataset(ctx, args, precision, runs)
+            status = ""
+            if outp is None:
+                cand = os.path.splitext(inp)[0] + ".out"
+                outp = cand if os.path.exists(cand) else None
+            if outp is not None:
+                expected = futhark_values.load(outp)[0]
+                ok, err = compare(got, expected, abs_tol)
+                status = f"max abs err {err:.2e} {'ok' if ok else 'FAILED'}"
+                all_ok &= ok
+            mean = float(np.mean(times))
+            rsd = float(np.std(times) / mean) if mean > 0 else 0.0
+            print(f"{name}: {mean:10.0f}us (RSD: {rsd:.3f}; min: {min(times)}us) {status}",
+                  file=out, flush=True)
+            results[name] = {"runtimes": times, "stderr": "" if not status.endswith("FAILED")
+                             else f"result differs from the expected output ({status})"}
+    return {PROGRAM_KEY: {"datasets": results}}, all_ok
+
+
+def main(argv: Optional[List[str]] = None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("datasets", nargs="*", help="dataset .in files")
+    ap.add_argument("--program", help="the reference's LocVolCalib.fut: run its stanza's datasets")
+    ap.add_argument("--runs", type=int, default=10)
+    ap.add_argument("--precision", choices=("fp32", "fp32r", "fp64"), default="fp32r")
+    ap.add_argument("--tolerance", type=float, default=5e-6)
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--json", help="write the futhark-bench JSON here")
+    a = ap.parse_args(argv)
+    pairs: List[Tuple[str, Optional[str]]] = []
+    names: List[str] = []
+    if a.program:
+        base = os.path.dirname(os.path.abspath(a.program))
+        with open(a.program) as f:
+            for inp, outp in parse_stanza(f.read()):
+                pairs.append((os.path.join(base, inp),
+                              os.path.join(base, outp) if outp else None))
+                names.append(inp)
+    for d in a.datasets:
+        pairs.append((d, None))
+        names.append(d)
+    if not pairs:
+        ap.error("no datasets: give .in files or --program")
+    prec = {"fp32": 0, "fp32r": 1, "fp64": 2}[a.precision]
+    res, ok = bench(pairs, runs=a.runs, precision=prec, abs_tol=a.tolerance,
+                    device=a.device, names=names)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(res, f, indent=1)
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

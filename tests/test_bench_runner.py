@@ -1,0 +1,66 @@
+"""The futhark-bench-shaped dataset runner: stanza parsing, argument
+mapping and validation on the CPU; a real run over the reference's three
+datasets on the GPU."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from locvolcalib import bench_runner, futhark_values
+
+# The reference's stanza (LocVolCalib.fut:1-10), as data.
+STANZA = """-- LocVolCalib
+-- ==
+-- compiled input @ LocVolCalib-data/small.in
+-- output @ LocVolCalib-data/small.out
+--
+-- compiled input @ LocVolCalib-data/medium.in
+-- output @ LocVolCalib-data/medium.out
+--
+-- compiled input @ LocVolCalib-data/large.in
+-- output @ LocVolCalib-data/large.out
+
+def initGrid (s0: f32) = s0
+"""
+
+
+def test_parse_stanza():
+    pairs = bench_runner.parse_stanza(STANZA)
+    assert pairs == [("LocVolCalib-data/%s.in" % n, "LocVolCalib-data/%s.out" % n)
+                     for n in ("small", "medium", "large")]
+    assert bench_runner.parse_stanza("def f = 1") == []
+
+
+def test_main_arguments_from_datasets():
+    args = bench_runner.main_arguments(
+        futhark_values.load(os.path.join(GOLDEN, "LocVolCalib-data", "small.in")))
+    assert args[:4] == (16, 32, 256, 256)
+    assert np.allclose(args[4:], (0.03, 5.0, 0.2, 0.6, 0.5))
+    with pytest.raises(ValueError):
+        bench_runner.main_arguments([np.int32(1)])
+
+
+def test_compare():
+    ok, err = bench_runner.compare(np.array([1.0, 2.0]), np.array([1.0, 2.0 + 1e-6]), 5e-6)
+    assert ok and err < 2e-6
+    ok, _ = bench_runner.compare(np.array([1.0]), np.array([1.1]), 5e-6)
+    assert not ok
+    ok, err = bench_runner.compare(np.zeros(2), np.zeros(3), 5e-6)
+    assert not ok and err == float("inf")
+
+
+@pytest.mark.gpu
+def test_runner_on_reference_datasets(tmp_path):
+    print("case: bench runner on small/medium/large", flush=True)
+    ds = [(os.path.join(GOLDEN, "LocVolCalib-data", n + ".in"), None)
+          for n in ("small", "medium", "large")]
+    res, ok = bench_runner.bench(ds, runs=2, names=["small", "medium", "large"])
+    assert ok
+    sets = res[bench_runner.PROGRAM_KEY]["datasets"]
+    assert set(sets) == {"small", "medium", "large"}
+    assert all(len(v["runtimes"]) == 2 and min(v["runtimes"]) > 0 for v in sets.values())
+    out = tmp_path / "r.json"
+    assert bench_runner.main([ds[0][0], "--runs", "1", "--json", str(out)]) == 0
+    assert bench_runner.PROGRAM_KEY in json.loads(out.read_text())
