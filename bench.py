@@ -145,6 +145,38 @@ def bench_batched(lvc, reps=3, n=512, seed=42):
             "grid_point_steps_per_s": round(elems / dt, 1)}
 
 
+def bench_ocean(lvc, n=57600, m=115, reps=20, seed=3):
+    """The ocean-sim tridiagonal entries (misc/ocean-sim/tridiag-test.fut) on
+    their `random input` shape, [57600][115] f32 x 4 (values uniform in
+    [0, 1), as futhark's random inputs), inputs resident on the device:
+    ms per call and algorithmic bytes (read a, b, c, y, write the result:
+    5*n*m*4) per second against 8 TB/s."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrs = [rng.random((n, m), dtype=np.float32) for _ in range(4)]
+    out = {"shape": [n, m], "seed": seed}
+    with lvc.LocVolCalib(device=0) as ctx:
+        hs = [ctx.upload_f32_2d(v) for v in arrs]
+        try:
+            for seq, key in ((True, "tridagNestedSeq"), (False, "tridagNested")):
+                for _ in range(3):
+                    ctx.free_f32_2d(ctx.tridag_nested_raw(*hs, seq=seq))
+                ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    h = ctx.tridag_nested_raw(*hs, seq=seq)
+                    ctx.sync()
+                    ctx.free_f32_2d(h)
+                dt = (time.perf_counter() - t0) / reps
+                gbs = 5 * n * m * 4 / dt / 1e9
+                out[key] = {"ms_per_call": round(dt * 1e3, 4), "achieved_GBs": round(gbs, 1),
+                            "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
+        finally:
+            for h in hs:
+                ctx.free_f32_2d(h)
+    return out
+
+
 def bench_serving(lvc, rate, duration_s=1.5, seed=7, max_slots=1024, budget=64 << 30):
     """Poisson arrivals at `rate` req/s of single problems (sizes and
     parameters as the batched set), continuous batching, FP32 + refinement.
@@ -291,6 +323,7 @@ def main():
                      "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3)},
     }
     if not args.no_forms:
+        out["ocean_tridiag"] = bench_ocean(lvc)
         out["batched"] = bench_batched(lvc)
         out["serving"] = [bench_serving(lvc, float(r)) for r in args.serve_rates.split(",")]
     if not args.no_cpu_baseline:

@@ -1,0 +1,164 @@
+// HIP kernels for gfx950: the batched tridiagonal solves of the ocean-sim
+// test program (misc/ocean-sim/tridiag-test.fut:14-26), [n][m] f32 systems,
+// one system per row.
+//
+//   tridag_seq_kernel  tridagSeq (misc/ocean-sim/tridiag.fut:81-95): Thomas.
+//   tridag_par_kernel  tridagPar (tridiag.fut:4-79): the three recurrences
+//                      in the form of the reference's scans, evaluated left to
+//                      right.
+//
+// One thread per system, in the reference's order of operations with IEEE
+// division and no contraction (the file is built with -ffp-contract=off), so
+// the results are the bits of a sequential f32 evaluation: these systems are
+// not diagonally dominant, and a reordered (partitioned or cyclic-reduction)
+// solve would meet pivots the reference's never does.  The forward pass keeps
+// its per-row values for the backward pass in LDS (c' or the modified
+// diagonal, one column per lane) when m <= kLdsRows, otherwise in a global
+// scratch array; y' goes to the output array in place.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace {
+
+constexpr int kThreads = 64;   // systems per workgroup (one wavefront)
+constexpr int kLdsRows = 256;  // longest system kept in LDS: 64 KiB
+
+// Per-row storage of the forward pass: LDS column `lane` (stride kThreads,
+// conflict-free) or the system's own row of a global scratch array.
+struct RowStore {
+  float *p;
+  int stride;
+  __device__ float &operator[](int i) const { return p[i * stride]; }
+};
+
+// tridiag.fut:81-95
+template <bool LDS>
+__global__ __launch_bounds__(kThreads) void tridag_seq_kernel(
+    const float *__restrict__ a, const float *__restrict__ b,
+    const float *__restrict__ c, const float *__restrict__ y,
+    float *__restrict__ out, float *__restrict__ scratch, int64_t n, int m) {
+  extern __shared__ float lds[];  // m * kThreads floats when LDS
+  const int64_t sys = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (sys >= n) return;
+  const int64_t off = sys * m;
+  const float *A = a + off, *B = b + off, *C = c + off, *Yv = y + off;
+  float *X = out + off;
+  const RowStore cp = LDS ? RowStore{lds + threadIdx.x, kThreads} : RowStore{scratch + off, 1};
+  float cprev = C[0] / B[0];
+  float yprev = Yv[0] / B[0];
+  cp[0] = cprev;
+  X[0] = yprev;
+  for (int i = 1; i < m; ++i) {
+    const float ai = A[i];
+    const float norm = 1.0f / (B[i] - ai * cprev);
+    cprev = C[i] * norm;
+    yprev = (Yv[i] - ai * yprev) * norm;
+    cp[i] = cprev;
+    X[i] = yprev;
+  }
+  float next = yprev;  // solution[m-1] = yp[m-1]
+  for (int i = m - 2; i >= 0; --i) {
+    next = X[i] - cp[i] * next;
+    X[i] = next;
+  }
+}
+
+// tridiag.fut:4-79: recurrence 1 as the scan of normalised 2x2 matrices,
+// recurrences 2 and 3 as scans of linear-function composition, in the
+// operator forms of the reference, accumulated left to right.
+template <bool LDS>
+__global__ __launch_bounds__(kThreads) void tridag_par_kernel(
+    const float *__restrict__ a, const float *__restrict__ b,
+    const float *__restrict__ c, const float *__restrict__ y,
+    float *__restrict__ out, float *__restrict__ scratch, int64_t n, int m) {
+  extern __shared__ float lds[];  // m * kThreads floats when LDS
+  const int64_t sys = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (sys >= n) return;
+  const int64_t off = sys * m;
+  const float *A = a + off, *B = b + off, *C = c + off, *Yv = y + off;
+  float *X = out + off;
+  const RowStore bb = LDS ? RowStore{lds + threadIdx.x, kThreads} : RowStore{scratch + off, 1};
+  // Recurrence 1 (:8-31): b'_i = (t0*b0 + t1) / (t2*b0 + t3) from the scan
+  // of (b_i, -a_i*c_{i-1}, 1, 0) under the normalised product.
+  const float b0 = B[0];
+  float t0 = 1.0f, t1 = 0.0f, t2 = 0.0f, t3 = 1.0f;  // scan of the identity
+  bb[0] = (t0 * b0 + t1) / (t2 * b0 + t3);
+  for (int i = 1; i < m; ++i) {
+    const float m0 = B[i], m1 = 0.0f - A[i] * C[i - 1], m2 = 1.0f, m3 = 0.0f;
+    const float value = 1.0f / (t0 * m0);
+    const float n0 = (m0 * t0 + m1 * t2) * value;
+    const float n1 = (m0 * t1 + m1 * t3) * value;
+    const float n2 = (m2 * t0 + m3 * t2) * value;
+    const float n3 = (m2 * t1 + m3 * t3) * value;
+    t0 = n0;
+    t1 = n1;
+    t2 = n2;
+    t3 = n3;
+    bb[i] = (t0 * b0 + t1) / (t2 * b0 + t3);
+  }
+  // Recurrence 2 (:36-53): y'_i = f0 + f1*y0 from the scan of
+  // (y_i, -a_i/b'_{i-1}) under (b0 + b1*a0, a1*b1).
+  const float y0 = Yv[0];
+  float f0 = 0.0f, f1 = 1.0f;
+  X[0] = f0 + f1 * y0;
+  for (int i = 1; i < m; ++i) {
+    const float l0 = Yv[i], l1 = 0.0f - A[i] / bb[i - 1];
+    const float g0 = l0 + l1 * f0, g1 = f1 * l1;
+    f0 = g0;
+    f1 = g1;
+    X[i] = f0 + f1 * y0;
+  }
+  // Recurrence 3 (:57-78): from yn = y'_{m-1}/b'_{m-1}, the reversed scan of
+  // (y'_i/b'_i, -c_i/b'_i); the final reverse is the store index.
+  const float yn = X[m - 1] / bb[m - 1];
+  f0 = 0.0f;
+  f1 = 1.0f;
+  X[m - 1] = f0 + f1 * yn;
+  for (int k = 1; k < m; ++k) {
+    const int i = m - k - 1;
+    const float bi = bb[i];
+    const float l0 = X[i] / bi, l1 = 0.0f - C[i] / bi;
+    const float g0 = l0 + l1 * f0, g1 = f1 * l1;
+    f0 = g0;
+    f1 = g1;
+    X[i] = f0 + f1 * yn;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// form: 0 = tridagPar (scan form), 1 = tridagSeq (Thomas).  scratch: n*m
+// floats, used only when m > the LDS limit (may be NULL otherwise).
+int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
+                      const float *c, const float *y, float *out, float *scratch,
+                      int64_t n, int m) {
+  if (n <= 0) return 0;
+  if (m <= 0) return (int)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t blocks = (n + kThreads - 1) / kThreads;
+  if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
+  const bool lds = m <= kLdsRows;
+  if (!lds && !scratch) return (int)hipErrorInvalidValue;
+  const size_t shm = lds ? (size_t)m * kThreads * sizeof(float) : 0;
+  if (form == 1) {
+    if (lds)
+      tridag_seq_kernel<true><<<(unsigned)blocks, kThreads, shm, st>>>(a, b, c, y, out, scratch, n, m);
+    else
+      tridag_seq_kernel<false><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, scratch, n, m);
+  } else if (form == 0) {
+    if (lds)
+      tridag_par_kernel<true><<<(unsigned)blocks, kThreads, shm, st>>>(a, b, c, y, out, scratch, n, m);
+    else
+      tridag_par_kernel<false><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, scratch, n, m);
+  } else {
+    return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
+}
+
+int lvc_tridag_lds_rows(void) { return kLdsRows; }
+
+}  // extern "C"

@@ -95,6 +95,16 @@ _SIGNATURES = {
     "futhark_entry_main_f64": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _i32, _i32, _i32, _i32,
                                               _f64, _f64, _f64, _f64, _f64]),
     "futhark_entry_main_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 9 + [_i32]),
+    "futhark_new_f32_2d": (_vp, [_vp, _vp, _i64, _i64]),
+    "futhark_new_raw_f32_2d": (_vp, [_vp, _vp, _i64, _i64]),
+    "futhark_free_f32_2d": (ctypes.c_int, [_vp, _vp]),
+    "futhark_values_f32_2d": (ctypes.c_int, [_vp, _vp, _vp]),
+    "futhark_values_raw_f32_2d": (_vp, [_vp, _vp]),
+    "futhark_shape_f32_2d": (ctypes.POINTER(_i64), [_vp, _vp]),
+    "futhark_entry_tridagNested": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),
+    "futhark_entry_tridagNestedConst": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),
+    "futhark_entry_tridagNestedSeq": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),
+    "futhark_entry_tridagNestedSeqConst": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),
     "lvc_server_new": (_vp, [_vp, _i32, _i64, _i32]),
     "lvc_server_free": (None, [_vp]),
     "lvc_server_submit": (_i64, [_vp, ctypes.POINTER(_Request)]),
@@ -293,6 +303,57 @@ class LocVolCalib:
             return self._values("f64", h)
         finally:
             self._free("f64", h)
+
+    def upload_f32_2d(self, v) -> int:
+        """A device [n][m] f32 array (handle) holding v; free with free_f32_2d."""
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        h = self._L.futhark_new_f32_2d(self._ctx, v.ctypes.data, v.shape[0], v.shape[1])
+        if not h:
+            self._check(1, "futhark_new_f32_2d")
+        return h
+
+    def free_f32_2d(self, h: int) -> None:
+        self._L.futhark_free_f32_2d(self._ctx, h)
+
+    def tridag_nested_raw(self, ha: int, hb: int, hc: int, hy: int, seq: bool = False,
+                          const: bool = False) -> int:
+        """Enqueue a tridagNested* entry on device arrays; returns the result
+        handle (asynchronous)."""
+        name = "futhark_entry_tridagNested" + ("Seq" if seq else "") + ("Const" if const else "")
+        out = _vp()
+        self._check(getattr(self._L, name)(self._ctx, ctypes.byref(out), ha, hb, hc, hy), name)
+        return out.value
+
+    def tridag_nested(self, a, b, c, y, seq: bool = False, const: bool = False) -> np.ndarray:
+        """misc/ocean-sim/tridiag-test.fut:14-26: solves the n tridiagonal
+        systems in the rows of the [n][m] f32 arrays a, b, c, y (entry
+        tridagNested, or ...Seq for Thomas, ...Const for m = 115)."""
+        arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (a, b, c, y)]
+        if any(v.ndim != 2 for v in arrs):
+            raise ValueError("tridag_nested takes [n][m] arrays")
+        name = "futhark_entry_tridagNested" + ("Seq" if seq else "") + ("Const" if const else "")
+        L = self._L
+        handles = []
+        try:
+            for v in arrs:
+                h = L.futhark_new_f32_2d(self._ctx, v.ctypes.data, v.shape[0], v.shape[1])
+                if not h:
+                    self._check(1, "futhark_new_f32_2d")
+                handles.append(h)
+            out = _vp()
+            self._check(getattr(L, name)(self._ctx, ctypes.byref(out), *handles), name)
+            try:
+                shape = L.futhark_shape_f32_2d(self._ctx, out)
+                res = np.empty((shape[0], shape[1]), np.float32)
+                if res.size:
+                    self._check(L.futhark_values_f32_2d(self._ctx, out, res.ctypes.data),
+                                "futhark_values_f32_2d")
+                return res
+            finally:
+                L.futhark_free_f32_2d(self._ctx, out)
+        finally:
+            for h in handles:
+                L.futhark_free_f32_2d(self._ctx, h)
 
     def main_batch(self, numX: Sequence[int], numY: Sequence[int], numT: Sequence[int],
                    s0, strike, t, alpha, nu, beta,

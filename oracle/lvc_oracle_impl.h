@@ -174,6 +174,61 @@ void FN(lvc_tridag_seq_)(const R *a, const R *b, const R *c, const R *y,
     out[i] = yp[i] - cp[i] * out[i + 1];
 }
 
+/* misc/ocean-sim/tridiag.fut:4-79 (the ocean-sim program's tridagPar): the
+ * same three recurrences as LocVolCalib.fut:68-124, but recurrence 1 is the
+ * scan of un-normalised 2x2 matrices (b_i, -a_i*c_{i-1}, 1, 0) under the
+ * product scaled by 1/(a0*b0) (:8-31), a different operator form.  Evaluated
+ * left to right.  `out` may alias `y`.  `tmp` holds 5*n scratch values. */
+void FN(lvc_tridag_par4_)(const R *a, const R *b, const R *c, const R *y,
+                          int64_t n, R *out, R *tmp) {
+  R *bb = tmp, *yy = tmp + n, *s = tmp + 2 * n;
+  const R b0 = b[0];
+  R t0 = (R)1.0, t1 = (R)0.0, t2 = (R)0.0, t3 = (R)1.0; /* identity (:11) */
+  for (int64_t i = 0; i < n; i++) {
+    if (0 < i) {
+      const R m0 = b[i], m1 = (R)0.0 - a[i] * c[i - 1], m2 = (R)1.0, m3 = (R)0.0;
+      const R value = (R)1.0 / (t0 * m0);
+      const R n0 = (m0 * t0 + m1 * t2) * value;
+      const R n1 = (m0 * t1 + m1 * t3) * value;
+      const R n2 = (m2 * t0 + m3 * t2) * value;
+      const R n3 = (m2 * t1 + m3 * t3) * value;
+      t0 = n0;
+      t1 = n1;
+      t2 = n2;
+      t3 = n3;
+    }
+    bb[i] = (t0 * b0 + t1) / (t2 * b0 + t3);
+  }
+  /* Recurrences 2 and 3 (:36-78): the linear-function scans of
+   * LocVolCalib.fut:96-123, same operators. */
+  const R y0 = y[0];
+  R f0 = (R)0.0, f1 = (R)1.0;
+  for (int64_t i = 0; i < n; i++) {
+    if (0 < i) {
+      const R l0 = y[i], l1 = (R)0.0 - a[i] / bb[i - 1];
+      const R g0 = l0 + l1 * f0, g1 = f1 * l1;
+      f0 = g0;
+      f1 = g1;
+    }
+    yy[i] = f0 + f1 * y0;
+  }
+  const R yn = yy[n - 1] / bb[n - 1];
+  f0 = (R)0.0;
+  f1 = (R)1.0;
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t i = n - k - 1;
+    if (0 < k) {
+      const R l0 = yy[i] / bb[i], l1 = (R)0.0 - c[i] / bb[i];
+      const R g0 = l0 + l1 * f0, g1 = f1 * l1;
+      f0 = g0;
+      f1 = g1;
+    }
+    s[i] = f0 + f1 * yn;
+  }
+  for (int64_t i = 0; i < n; i++)
+    out[i] = s[i];
+}
+
 /* LocVolCalib.fut:126-153 (explicitMethod) on `rows` rows of length m:
  * D, DD are [m][3]; mu, var, res, out are [rows][m]. */
 void FN(lvc_explicit_)(const R *D, const R *DD, const R *mu, const R *var,

@@ -22,6 +22,9 @@ tridag_par             LocVolCalib.fut:68-124                 dense solve (numpy
                                                               n <= 16; agreement with tridag_seq;
                                                               ocean-sim golden vectors (tridiag32-small)
 tridag_seq             misc/ocean-sim/tridiag.fut:81-95       dense solve; ocean-sim golden vectors
+                                                              (bit-exact)
+tridag_par4            misc/ocean-sim/tridiag.fut:4-79        dense solve for n <= 16; agreement with
+                                                              tridag_seq; ocean-sim golden vectors
 explicit_method        LocVolCalib.fut:126-153                dense matrix-vector product on tiny grids
 value / main (f32)     LocVolCalib.fut:211-252                the reference's fixtures small/medium/large
                                                               (.in -> .out), closed form price = s0 - K
@@ -85,7 +88,7 @@ def lib() -> ctypes.CDLL:
             f = getattr(_lib, "lvc_update_params" + suf)
             f.restype = None
             f.argtypes = [_ptr, _ptr, _c_i64, _c_i64, real, real, real, _ptr]
-            for name in ("lvc_tridag_par", "lvc_tridag_seq"):
+            for name in ("lvc_tridag_par", "lvc_tridag_seq", "lvc_tridag_par4"):
                 f = getattr(_lib, name + suf)
                 f.restype = None
                 f.argtypes = [_ptr] * 4 + [_c_i64, _ptr, _ptr]
@@ -220,6 +223,12 @@ def tridag_par(a, b, c, y) -> np.ndarray:
 def tridag_seq(a, b, c, y) -> np.ndarray:
     """misc/ocean-sim/tridiag.fut:81-95, Thomas."""
     return _tridag("lvc_tridag_seq", a, b, c, y)
+
+
+def tridag_par4(a, b, c, y) -> np.ndarray:
+    """misc/ocean-sim/tridiag.fut:4-79, the ocean-sim program's scan form
+    (un-normalised 2x2 matrices in recurrence 1)."""
+    return _tridag("lvc_tridag_par4", a, b, c, y)
 
 
 def explicit_method(D, DD, mu, var, res) -> np.ndarray:

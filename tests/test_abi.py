@@ -1,7 +1,8 @@
-"""The C ABI library: it loads, exports every function include/locvolcalib.h
+"""The C ABI library: it loads, exports every function include/*.h
 declares, and the parts that need no GPU (configuration, argument
 validation, error reporting) behave like the generated Futhark API."""
 import ctypes
+import glob
 import os
 import re
 
@@ -10,13 +11,14 @@ import pytest
 from conftest import ROOT
 import locvolcalib as lvc
 
-HEADER = os.path.join(ROOT, "include", "locvolcalib.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def header_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = set(re.findall(r"\b((?:futhark|lvc)_[a-z0-9_]+)\s*\(", text))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b((?:futhark|lvc)_[A-Za-z0-9_]+)\s*\(", text))
     return sorted(names)
 
 

@@ -230,6 +230,7 @@ def check_tridag_dense():
         ref = np.linalg.solve(_dense(a, b, c), y)
         np.testing.assert_allclose(oracle.tridag_par(a, b, c, y), ref, rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(oracle.tridag_seq(a, b, c, y), ref, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(oracle.tridag_par4(a, b, c, y), ref, rtol=1e-12, atol=1e-14)
 
 
 def test_tridag_dense_solve():
@@ -240,6 +241,8 @@ def test_tridag_par_vs_seq_long():
     rng = np.random.default_rng(3)
     a, b, c, y = _dd_system(rng, 256)
     np.testing.assert_allclose(oracle.tridag_par(a, b, c, y), oracle.tridag_seq(a, b, c, y),
+                               rtol=1e-12)
+    np.testing.assert_allclose(oracle.tridag_par4(a, b, c, y), oracle.tridag_seq(a, b, c, y),
                                rtol=1e-12)
 
 
@@ -253,10 +256,11 @@ def test_tridag_ocean_sim_golden():
     assert a.dtype == np.float32 and a.shape == (100, 115)
     seq = np.stack([oracle.tridag_seq(a[i], b[i], c[i], y[i]) for i in range(100)])
     assert np.array_equal(seq, out)
-    par = np.stack([oracle.tridag_par(a[i], b[i], c[i], y[i]) for i in range(100)])
-    rel = np.abs(par.astype(np.float64) - out) / np.abs(out)
-    assert np.median(rel) < 1e-5
-    assert np.percentile(rel, 99) < 1e-2
+    for fn in (oracle.tridag_par, oracle.tridag_par4):
+        par = np.stack([fn(a[i], b[i], c[i], y[i]) for i in range(100)])
+        rel = np.abs(par.astype(np.float64) - out) / np.abs(out)
+        assert np.median(rel) < 1e-5
+        assert np.percentile(rel, 99) < 1e-2
 
 
 # ---- explicitMethod (LocVolCalib.fut:126-153) ------------------------------
