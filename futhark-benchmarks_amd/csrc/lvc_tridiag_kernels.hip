@@ -7,7 +7,8 @@
 //                      in the form of the reference's scans, evaluated left to
 //                      right.
 //
-// One thread per system, in the reference's order of operations with IEEE
+// One thread per system (systems of up to 128 unknowns through the staged
+// kernel below), in the reference's order of operations with IEEE
 // division and no contraction (the file is built with -ffp-contract=off), so
 // the results are the bits of a sequential f32 evaluation: these systems are
 // not diagonally dominant, and a reordered (partitioned or cyclic-reduction)
@@ -126,6 +127,184 @@ __global__ __launch_bounds__(kThreads) void tridag_par_kernel(
   }
 }
 
+// Staged variant for systems of up to kStagedRows unknowns: one wavefront per
+// 64 systems.  Chunks of kChunk columns of a, b, c, y are loaded into LDS
+// with coalesced row reads (each system's row is contiguous) and read back
+// one system per lane.  c' (or the scan form's l1) waits in LDS; y' (or l0)
+// goes to the output array in place through the same transpose, and the
+// backward pass reads it back chunk by chunk in reverse.  The arithmetic and
+// its order per system are those of the kernels above (recurrences 1-3 of
+// the scan form advance together: each step uses only earlier steps' values,
+// so interleaving them changes no result).
+constexpr int kStagedRows = 128;
+constexpr int kChunk = 16;
+constexpr int kRowStep = kThreads / kChunk;  // rows per staging instruction
+
+struct Stage {
+  float v[4][kThreads][kChunk + 1];  // a, b, c, y (y's slot: y' or l0 after use)
+};
+
+// The same rows and columns loaded into registers (this lane's slots), to be
+// written to LDS later: the next chunk's loads fly while this one computes.
+__device__ __forceinline__ void fetch(float (&reg)[kThreads / kRowStep],
+                                      const float *__restrict__ src, int64_t s0, int rows,
+                                      int m, int i0, int lane) {
+  const int sr = lane / kChunk, sc = lane % kChunk, i = i0 + sc;
+#pragma unroll
+  for (int q = 0; q < kThreads / kRowStep; ++q) {
+    const int r = q * kRowStep + sr;
+    reg[q] = (r < rows && i < m) ? src[(s0 + r) * m + i] : 0.0f;
+  }
+}
+
+__device__ __forceinline__ void deposit(float (&st)[kThreads][kChunk + 1],
+                                        const float (&reg)[kThreads / kRowStep], int lane) {
+  const int sr = lane / kChunk, sc = lane % kChunk;
+#pragma unroll
+  for (int q = 0; q < kThreads / kRowStep; ++q) st[q * kRowStep + sr][sc] = reg[q];
+}
+
+__device__ __forceinline__ void stage_out(const float (&st)[kThreads][kChunk + 1],
+                                          float *__restrict__ dst, int64_t s0, int rows,
+                                          int m, int i0, int lane) {
+  const int sr = lane / kChunk, sc = lane % kChunk, i = i0 + sc;
+#pragma unroll
+  for (int q = 0; q < kThreads / kRowStep; ++q) {
+    const int r = q * kRowStep + sr;
+    if (r < rows && i < m) dst[(s0 + r) * m + i] = st[r][sc];
+  }
+}
+
+template <bool SEQ>
+__global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
+    const float *__restrict__ a, const float *__restrict__ b,
+    const float *__restrict__ c, const float *__restrict__ y,
+    float *__restrict__ out, int64_t n, int m) {
+  __shared__ Stage S;
+  __shared__ float fw[kStagedRows][kThreads];  // c' or l1, column = lane
+  const int lane = threadIdx.x;
+  const int64_t s0 = (int64_t)blockIdx.x * kThreads;
+  const int rows = (int)min((int64_t)kThreads, n - s0);
+  const int nch = (m + kChunk - 1) / kChunk;
+  // Thomas carries
+  float cprev = 0.0f, yprev = 0.0f;
+  // scan-form carries
+  float t0 = 1.0f, t1 = 0.0f, t2 = 0.0f, t3 = 1.0f, b0 = 0.0f, bprev = 0.0f;
+  float f0 = 0.0f, f1 = 1.0f, y0 = 0.0f, last_bb = 1.0f, last_yy = 0.0f;
+
+  constexpr int kQ = kThreads / kRowStep;
+  float pa[kQ], pb[kQ], pc[kQ], py[kQ];  // the next chunk, in flight
+  fetch(pa, a, s0, rows, m, 0, lane);
+  fetch(pb, b, s0, rows, m, 0, lane);
+  fetch(pc, c, s0, rows, m, 0, lane);
+  fetch(py, y, s0, rows, m, 0, lane);
+  for (int ch = 0; ch < nch; ++ch) {
+    const int i0 = ch * kChunk;
+    deposit(S.v[0], pa, lane);
+    deposit(S.v[1], pb, lane);
+    deposit(S.v[2], pc, lane);
+    deposit(S.v[3], py, lane);
+    __syncthreads();
+    if (ch + 1 < nch) {
+      fetch(pa, a, s0, rows, m, i0 + kChunk, lane);
+      fetch(pb, b, s0, rows, m, i0 + kChunk, lane);
+      fetch(pc, c, s0, rows, m, i0 + kChunk, lane);
+      fetch(py, y, s0, rows, m, i0 + kChunk, lane);
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = i0 + j;
+      if (i < m) {
+        const float ai = S.v[0][lane][j], bi = S.v[1][lane][j];
+        const float ci = S.v[2][lane][j], yi = S.v[3][lane][j];
+        if (SEQ) {
+          if (i == 0) {
+            cprev = ci / bi;
+            yprev = yi / bi;
+          } else {
+            const float norm = 1.0f / (bi - ai * cprev);
+            cprev = ci * norm;
+            yprev = (yi - ai * yprev) * norm;
+          }
+          fw[i][lane] = cprev;
+          S.v[3][lane][j] = yprev;
+        } else {
+          // recurrence 1 (c_{i-1} carried in cprev)
+          if (i == 0) {
+            b0 = bi;
+          } else {
+            const float m0 = bi, m1 = 0.0f - ai * cprev, m2 = 1.0f, m3 = 0.0f;
+            const float value = 1.0f / (t0 * m0);
+            const float n0 = (m0 * t0 + m1 * t2) * value;
+            const float n1 = (m0 * t1 + m1 * t3) * value;
+            const float n2 = (m2 * t0 + m3 * t2) * value;
+            const float n3 = (m2 * t1 + m3 * t3) * value;
+            t0 = n0;
+            t1 = n1;
+            t2 = n2;
+            t3 = n3;
+          }
+          const float bb = (t0 * b0 + t1) / (t2 * b0 + t3);
+          // recurrence 2
+          float yy;
+          if (i == 0) {
+            y0 = yi;
+            yy = f0 + f1 * y0;
+          } else {
+            const float l0 = yi, l1 = 0.0f - ai / bprev;
+            const float g0 = l0 + l1 * f0, g1 = f1 * l1;
+            f0 = g0;
+            f1 = g1;
+            yy = f0 + f1 * y0;
+          }
+          // this row's element of recurrence 3
+          S.v[3][lane][j] = yy / bb;
+          fw[i][lane] = 0.0f - ci / bb;
+          bprev = bb;
+          cprev = ci;
+          last_bb = bb;
+          last_yy = yy;
+        }
+      }
+    }
+    __syncthreads();
+    stage_out(S.v[3], out, s0, rows, m, i0, lane);
+    __syncthreads();
+  }
+
+  // backward pass over the chunks in reverse; the solution replaces y' / l0
+  float xnext = yprev;                                  // Thomas: x_{m-1} = y'_{m-1}
+  const float yn = SEQ ? 0.0f : last_yy / last_bb;      // scan form
+  float g0 = 0.0f, g1 = 1.0f;
+  fetch(py, out, s0, rows, m, (nch - 1) * kChunk, lane);
+  for (int ch = nch - 1; ch >= 0; --ch) {
+    const int i0 = ch * kChunk;
+    deposit(S.v[3], py, lane);
+    __syncthreads();
+    if (ch > 0) fetch(py, out, s0, rows, m, i0 - kChunk, lane);
+#pragma unroll
+    for (int j = kChunk - 1; j >= 0; --j) {
+      const int i = i0 + j;
+      if (i < m) {
+        if (i == m - 1) {
+          S.v[3][lane][j] = SEQ ? yprev : 0.0f + 1.0f * yn;
+        } else if (SEQ) {
+          xnext = S.v[3][lane][j] - fw[i][lane] * xnext;
+          S.v[3][lane][j] = xnext;
+        } else {
+          const float h0 = S.v[3][lane][j] + fw[i][lane] * g0, h1 = g1 * fw[i][lane];
+          g0 = h0;
+          g1 = h1;
+          S.v[3][lane][j] = g0 + g1 * yn;
+        }
+      }
+    }
+    __syncthreads();
+    stage_out(S.v[3], out, s0, rows, m, i0, lane);
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -140,6 +319,14 @@ int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
   hipStream_t st = (hipStream_t)stream;
   const int64_t blocks = (n + kThreads - 1) / kThreads;
   if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
+  if (form != 0 && form != 1) return (int)hipErrorInvalidValue;
+  if (m <= kStagedRows) {
+    if (form == 1)
+      tridag_staged_kernel<true><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, n, m);
+    else
+      tridag_staged_kernel<false><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, n, m);
+    return (int)hipGetLastError();
+  }
   const bool lds = m <= kLdsRows;
   if (!lds && !scratch) return (int)hipErrorInvalidValue;
   const size_t shm = lds ? (size_t)m * kThreads * sizeof(float) : 0;
