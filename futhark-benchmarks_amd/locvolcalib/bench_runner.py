@@ -1,24 +1,26 @@
-"""A ``futhark bench``-shaped runner for the LocVolCalib entry point.
+"""A ``futhark bench``-shaped runner for the library's entry points.
 
-Reads the datasets of the reference's benchmark stanza (LocVolCalib.fut:1-10:
-``compiled input @ LocVolCalib-data/small.in`` / ``output @ ...small.out``)
-in Futhark's text or binary value format, runs ``main`` on each one a number
-of times through the C ABI, checks the result against the expected output,
-and reports per-run wall times in microseconds -- on stdout as one line per
-dataset, and with ``--json FILE`` in the JSON layout ``futhark bench
---json`` writes (README.md:36-37)::
+Reads the datasets of a program's benchmark stanza -- LocVolCalib.fut:1-10
+(``compiled input @ LocVolCalib-data/small.in`` / ``output @ ...small.out``)
+or misc/ocean-sim/tridiag-test.fut:6-12 (its four entries on
+``data/tridiag32-small.in``) -- in Futhark's text or binary value format,
+runs each entry on each dataset a number of times through the C ABI, checks
+the result against the expected output, and reports per-run wall times in
+microseconds: on stdout as one line per dataset, and with ``--json FILE`` in
+the JSON layout ``futhark bench --json`` writes (README.md:36-37)::
 
     {"finpar/LocVolCalib.fut:main":
        {"datasets": {"LocVolCalib-data/small.in": {"runtimes": [...],
                                                     "stderr": ""}}}}
 
-A run is the entry call plus the context sync that ends it (inputs are
-scalars, the result stays on the device); one warm-up run precedes the timed
-ones, as ``futhark bench`` does.
+A run is the entry call plus the context sync that ends it, with the inputs
+already on the device and the result left there; one warm-up run precedes the
+timed ones, as ``futhark bench`` does.
 
 Usage::
 
     python -m locvolcalib.bench_runner --program path/to/finpar/LocVolCalib.fut
+    python -m locvolcalib.bench_runner --program path/to/misc/ocean-sim/tridiag-test.fut
     python -m locvolcalib.bench_runner path/to/small.in path/to/medium.in ...
 
 The expected output of ``X.in`` is ``X.out`` beside it (or the stanza's
@@ -41,6 +43,18 @@ import numpy as np
 from . import futhark_values
 
 PROGRAM_KEY = "finpar/LocVolCalib.fut:main"
+OCEAN_PROGRAM = "misc/ocean-sim/tridiag-test.fut"
+OCEAN_ENTRIES = ("tridagNested", "tridagNestedConst", "tridagNestedSeq", "tridagNestedSeqConst")
+
+
+def parse_entries(text: str) -> List[str]:
+    """Entry names of a stanza's ``entry:`` line (tridiag-test.fut:7); the
+    default is ``main``."""
+    for ln in text.splitlines():
+        m = re.match(r"\s*--\s*entry:\s*(.+)$", ln)
+        if m:
+            return m.group(1).split()
+    return ["main"]
 
 
 def parse_stanza(text: str) -> List[Tuple[str, Optional[str]]]:
@@ -141,6 +155,67 @@ def bench(datasets: List[Tuple[str, Optional[str]]], runs: int = 10, precision: 
     return {PROGRAM_KEY: {"datasets": results}}, all_ok
 
 
+def bench_ocean(datasets: List[Tuple[str, Optional[str]]], entries: List[str], runs: int = 10,
+                device: int = 0, names: Optional[List[str]] = None, out=sys.stdout,
+                abs_tol: float = 0.0) -> Tuple[Dict, bool]:
+    """The ocean-sim entries on [n][m] f32 datasets (four arrays a, b, c, y).
+    The Seq entries are checked for equality with the expected output (the
+    bits of a sequential f32 Thomas solve); the scan-form entries, which
+    associate differently, by median relative difference (< 1e-5)."""
+    from . import LocVolCalib
+
+    results: Dict[str, Dict] = {}
+    all_ok = True
+    with LocVolCalib(device=device) as ctx:
+        for entry in entries:
+            if entry not in OCEAN_ENTRIES:
+                raise ValueError(f"unknown entry {entry}")
+            seq, const = "Seq" in entry, entry.endswith("Const")
+            sets: Dict[str, Dict] = {}
+            for i, (inp, outp) in enumerate(datasets):
+                name = names[i] if names else inp
+                vals = futhark_values.load(inp)
+                if len(vals) != 4:
+                    raise ValueError(f"{entry} takes 4 arrays, the dataset has {len(vals)}")
+                hs = [ctx.upload_f32_2d(v) for v in vals]
+                try:
+                    ctx.free_f32_2d(ctx.tridag_nested_raw(*hs, seq=seq, const=const))
+                    ctx.sync()
+                    times = []
+                    for _ in range(runs):
+                        t0 = time.perf_counter()
+                        h = ctx.tridag_nested_raw(*hs, seq=seq, const=const)
+                        ctx.sync()
+                        times.append(int(round((time.perf_counter() - t0) * 1e6)))
+                        ctx.free_f32_2d(h)
+                finally:
+                    for h in hs:
+                        ctx.free_f32_2d(h)
+                status = ""
+                if outp is None:
+                    cand = os.path.splitext(inp)[0] + ".out"
+                    outp = cand if os.path.exists(cand) else None
+                if outp is not None:
+                    got = ctx.tridag_nested(*vals, seq=seq, const=const)
+                    expected = futhark_values.load(outp)[0]
+                    if seq:
+                        ok, err = compare(got, expected, abs_tol)
+                        status = f"max abs err {err:.2e} {'ok' if ok else 'FAILED'}"
+                    else:
+                        rel = np.abs(got.astype(np.float64) - expected) / np.abs(expected)
+                        med = float(np.median(rel)) if rel.size else 0.0
+                        ok = med < 1e-5
+                        status = f"median rel diff {med:.2e} {'ok' if ok else 'FAILED'}"
+                    all_ok &= ok
+                mean = float(np.mean(times))
+                print(f"{entry} {name}: {mean:10.0f}us (min: {min(times)}us) {status}",
+                      file=out, flush=True)
+                sets[name] = {"runtimes": times,
+                              "stderr": "" if not status.endswith("FAILED") else status}
+            results[f"{OCEAN_PROGRAM}:{entry}"] = {"datasets": sets}
+    return results, all_ok
+
+
 def main(argv: Optional[List[str]] = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("datasets", nargs="*", help="dataset .in files")
@@ -153,21 +228,27 @@ def main(argv: Optional[List[str]] = None) -> int:
     a = ap.parse_args(argv)
     pairs: List[Tuple[str, Optional[str]]] = []
     names: List[str] = []
+    entries = ["main"]
     if a.program:
         base = os.path.dirname(os.path.abspath(a.program))
         with open(a.program) as f:
-            for inp, outp in parse_stanza(f.read()):
-                pairs.append((os.path.join(base, inp),
-                              os.path.join(base, outp) if outp else None))
-                names.append(inp)
+            text = f.read()
+        entries = parse_entries(text)
+        for inp, outp in parse_stanza(text):
+            pairs.append((os.path.join(base, inp),
+                          os.path.join(base, outp) if outp else None))
+            names.append(inp)
     for d in a.datasets:
         pairs.append((d, None))
         names.append(d)
     if not pairs:
         ap.error("no datasets: give .in files or --program")
-    prec = {"fp32": 0, "fp32r": 1, "fp64": 2}[a.precision]
-    res, ok = bench(pairs, runs=a.runs, precision=prec, abs_tol=a.tolerance,
-                    device=a.device, names=names)
+    if entries != ["main"]:
+        res, ok = bench_ocean(pairs, entries, runs=a.runs, device=a.device, names=names)
+    else:
+        prec = {"fp32": 0, "fp32r": 1, "fp64": 2}[a.precision]
+        res, ok = bench(pairs, runs=a.runs, precision=prec, abs_tol=a.tolerance,
+                        device=a.device, names=names)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(res, f, indent=1)

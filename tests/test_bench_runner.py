@@ -64,3 +64,42 @@ def test_runner_on_reference_datasets(tmp_path):
     out = tmp_path / "r.json"
     assert bench_runner.main([ds[0][0], "--runs", "1", "--json", str(out)]) == 0
     assert bench_runner.PROGRAM_KEY in json.loads(out.read_text())
+
+
+# The ocean-sim stanza (misc/ocean-sim/tridiag-test.fut:6-12), as data.
+OCEAN_STANZA = """-- ==
+-- entry: tridagNested tridagNestedConst tridagNestedSeq tridagNestedSeqConst
+--
+-- compiled input @ data/tridiag32-small.in
+-- output @ data/tridiag32-small.out
+--
+-- compiled random input { [57600][115]f32 [57600][115]f32 [57600][115]f32 [57600][115]f32 }
+
+entry tridagNested [n] [m] (a: [n][m]DTYPE) = a
+"""
+
+
+def test_parse_entries():
+    assert bench_runner.parse_entries(OCEAN_STANZA) == list(bench_runner.OCEAN_ENTRIES)
+    assert bench_runner.parse_entries(STANZA) == ["main"]
+    assert bench_runner.parse_stanza(OCEAN_STANZA) == [("data/tridiag32-small.in",
+                                                        "data/tridiag32-small.out")]
+
+
+@pytest.mark.gpu
+def test_runner_on_ocean_dataset(tmp_path):
+    print("case: bench runner, four ocean-sim entries on tridiag32-small", flush=True)
+    base = os.path.join(GOLDEN, "ocean-sim")
+    ds = [(os.path.join(base, "tridiag32-small.in"), os.path.join(base, "tridiag32-small.out"))]
+    res, ok = bench_runner.bench_ocean(ds, list(bench_runner.OCEAN_ENTRIES), runs=2,
+                                       names=["data/tridiag32-small.in"])
+    assert ok
+    assert set(res) == {f"{bench_runner.OCEAN_PROGRAM}:{e}" for e in bench_runner.OCEAN_ENTRIES}
+    for v in res.values():
+        rt = v["datasets"]["data/tridiag32-small.in"]["runtimes"]
+        assert len(rt) == 2 and min(rt) > 0
+    prog = tmp_path / "tridiag-test.fut"
+    prog.write_text(OCEAN_STANZA.replace("data/", base + "/"))
+    out = tmp_path / "o.json"
+    assert bench_runner.main(["--program", str(prog), "--runs", "1", "--json", str(out)]) == 0
+    assert len(json.loads(out.read_text())) == 4
