@@ -91,14 +91,18 @@ __device__ __forceinline__ double dpp(double old, double v) {
 }
 
 // Lanes of one system: `seg` = this lane's index in its group of `G` lanes.
+// Groups of at most 16 lanes lie inside one DPP row and exchange values with
+// DPP row shifts; wider groups (32, 64 lanes) use ds_bpermute.  The choice is
+// a template parameter (ROW) of every routine that exchanges values, and a
+// tile picks its instantiation once, so the exchanges sit in straight-line
+// code rather than behind a branch each.
 struct Lanes {
   int seg, G;
-  bool row;   // G <= 16: the group lies inside one DPP row
   int lane4;  // 4 * (lane index in the wavefront): ds_bpermute address base
 };
 
 __device__ __forceinline__ Lanes make_lanes(int seg, int G) {
-  return Lanes{seg, G, G <= 16, (int)__lane_id() << 2};
+  return Lanes{seg, G, (int)__lane_id() << 2};
 }
 
 // ds_bpermute: the value of lane (addr >> 2) mod 64.
@@ -113,10 +117,10 @@ __device__ __forceinline__ double bperm(int addr, double v) {
 }
 
 // Value of lane seg-st of the group (ident below the group's first lane).
-template <typename T>
+template <bool ROW, typename T>
 __device__ __forceinline__ T from_below(const Lanes &L, T v, int st, T ident) {
   T r;
-  if (L.row) {
+  if constexpr (ROW) {
     switch (st) {
       case 1: r = dpp<0x111>(ident, v); break;  // row_shr:1
       case 2: r = dpp<0x112>(ident, v); break;
@@ -130,10 +134,10 @@ __device__ __forceinline__ T from_below(const Lanes &L, T v, int st, T ident) {
 }
 
 // Value of lane seg+st of the group (ident past the group's last lane).
-template <typename T>
+template <bool ROW, typename T>
 __device__ __forceinline__ T from_above(const Lanes &L, T v, int st, T ident) {
   T r;
-  if (L.row) {
+  if constexpr (ROW) {
     switch (st) {
       case 1: r = dpp<0x101>(ident, v); break;  // row_shl:1
       case 2: r = dpp<0x102>(ident, v); break;
@@ -316,6 +320,7 @@ struct SegSys {
 
   // a, c: taps of rows 0..M-1 of this segment; s: diagonal shift per row
   // (dtInv for rows of the system, 1 for padding rows past its end).
+  template <bool ROW>
   __device__ __forceinline__ void prepare(const T (&a)[M], const T (&cc)[M],
                                           const T (&s)[M], Lanes lanes) {
     L = lanes;
@@ -346,8 +351,8 @@ struct SegSys {
       w[k] = -c[k] * w[k + 1] * rb[k];
     }
     // Reduced row of this lane: A X[seg-1] + B X[seg] + C X[seg+1] = D.
-    const T vn = from_above(L, v[0], 1, T(0));
-    const T wn = from_above(L, w[0], 1, T(0));
+    const T vn = from_above<ROW>(L, v[0], 1, T(0));
+    const T wn = from_above<ROW>(L, w[0], 1, T(0));
     T A = aL * v[M - 2];
     T B = fma(-cL, T(1) - vn, fma(-aL, T(1) - w[M - 2], s[M - 1]));
     T C = cL * wn;
@@ -356,10 +361,10 @@ struct SegSys {
     for (int l = 0; l < kMaxLevels; ++l) {
       const int st = 1 << l;
       if (st < L.G) {
-        const T Am = from_below(L, A, st, T(0)), Bm = from_below(L, B, st, T(1));
-        const T Cm = from_below(L, C, st, T(0));
-        const T Ap = from_above(L, A, st, T(0)), Bp = from_above(L, B, st, T(1));
-        const T Cp = from_above(L, C, st, T(0));
+        const T Am = from_below<ROW>(L, A, st, T(0)), Bm = from_below<ROW>(L, B, st, T(1));
+        const T Cm = from_below<ROW>(L, C, st, T(0));
+        const T Ap = from_above<ROW>(L, A, st, T(0)), Bp = from_above<ROW>(L, B, st, T(1));
+        const T Cp = from_above<ROW>(L, C, st, T(0));
         const T q1 = A * rcp(Bm), q2 = C * rcp(Bp);
         k1[l] = q1;
         k2[l] = q2;
@@ -372,6 +377,7 @@ struct SegSys {
   }
 
   // Solves for one right-hand side d (rows 0..M-1 of this segment).
+  template <bool ROW>
   __device__ __forceinline__ void solve(const T (&d)[M], T (&x)[M]) const {
     x[0] = d[0];
 #pragma unroll
@@ -379,18 +385,18 @@ struct SegSys {
     x[M - 2] = x[M - 2] * rb[M - 2];
 #pragma unroll
     for (int k = M - 3; k >= 0; --k) x[k] = fma(-c[k], x[k + 1], x[k]) * rb[k];
-    const T un = from_above(L, x[0], 1, T(0));
+    const T un = from_above<ROW>(L, x[0], 1, T(0));
     T D = fma(-cL, un, fma(-aL, x[M - 2], d[M - 1]));
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
       const int st = 1 << l;
       if (st < L.G) {
-        const T Dm = from_below(L, D, st, T(0)), Dp = from_above(L, D, st, T(0));
+        const T Dm = from_below<ROW>(L, D, st, T(0)), Dp = from_above<ROW>(L, D, st, T(0));
         D = fma(-Dp, k2[l], fma(-Dm, k1[l], D));
       }
     }
     const T X = D * rB;
-    const T XL = from_below(L, X, 1, T(0));
+    const T XL = from_below<ROW>(L, X, 1, T(0));
 #pragma unroll
     for (int k = 0; k < M - 1; ++k) x[k] = fma(w[k], X, fma(v[k], XL, x[k]));
     x[M - 1] = X;
@@ -400,12 +406,12 @@ struct SegSys {
 // One refinement pass on the solution x of  s*x + a*(x[-1]-x) + c*(x[+1]-x) = d:
 // the residual in FP64, a second solve with the same factorisation, the
 // correction added with one rounding.
-template <typename T, int M>
+template <bool ROW, typename T, int M>
 __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
                                        const T (&a)[M], const T (&c)[M],
                                        const T (&s)[M], T (&x)[M]) {
-  const T xl = from_below(sys.L, x[M - 1], 1, T(0));
-  const T xr = from_above(sys.L, x[0], 1, T(0));
+  const T xl = from_below<ROW>(sys.L, x[M - 1], 1, T(0));
+  const T xr = from_above<ROW>(sys.L, x[0], 1, T(0));
   T r[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
@@ -416,7 +422,7 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
     r[k] = (T)((double)d[k] - ax);
   }
   T e[M];
-  sys.solve(r, e);
+  sys.template solve<ROW>(r, e);
   // x + e rounded once to T: the correctly rounded sum, which is what an FP64
   // addition rounded back to T gives (the exact sum of two floats fits a
   // double whenever their exponents are within 29 of each other).
@@ -436,7 +442,7 @@ constexpr int kXsweepLds = 2 * M * LVC_BLOCK;  // elements of T
 // NB = 1: single-problem tiles, the group loop resolved at compile time (fewer
 // registers); NB = 0: tile.cpl problems.  Both run the same arithmetic, so a
 // problem's result does not depend on which variant its batch used.
-template <typename T, int M, bool REF, int NB>
+template <typename T, int M, bool REF, int NB, bool ROW>
 __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
                                             const lvc_problem *__restrict__ probs,
                                             const double *__restrict__ geo,
@@ -511,7 +517,7 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
   T U[M];
   if (NB == 1) explicit_terms(U);
   SegSys<T, M> sys;
-  sys.prepare(a, c, s, make_lanes(seg, G));
+  sys.template prepare<ROW>(a, c, s, make_lanes(seg, G));
 
   const int nb = NB > 0 ? NB : tile.cpl;
   for (int b = 0; b < nb; ++b) {
@@ -522,12 +528,12 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
       if (kPrefetch && b + 1 < nb) load_rows(b + 1);
     }
     T x[M];
-    sys.solve(U, x);
+    sys.template solve<ROW>(U, x);
     if (REF) {
       T d[M];
 #pragma unroll
       for (int k = 0; k < M; ++k) d[k] = lds_u[k * LVC_BLOCK + threadIdx.x];
-      refine(sys, d, a, c, s, x);
+      refine<ROW>(sys, d, a, c, s, x);
     }
     T out[M];
 #pragma unroll
@@ -574,7 +580,11 @@ __global__ __launch_bounds__(LVC_BLOCK, (kXsweepWaves<T, M, REF, SHARE>)) void l
     }
     return;
   }
-  xsweep_tile<T, M, REF, SHARE ? 0 : 1>(tiles[blockIdx.x - nfac], probs, geo, R, Y, smem);
+  const lvc_tile tile = tiles[blockIdx.x - nfac];
+  if (tile.lanes <= 16)
+    xsweep_tile<T, M, REF, SHARE ? 0 : 1, true>(tile, probs, geo, R, Y, smem);
+  else
+    xsweep_tile<T, M, REF, SHARE ? 0 : 1, false>(tile, probs, geo, R, Y, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -588,7 +598,7 @@ __global__ __launch_bounds__(LVC_BLOCK, (kXsweepWaves<T, M, REF, SHARE>)) void l
 // ---------------------------------------------------------------------------
 
 // Every thread of the workgroup must call this (it holds barriers).
-template <typename T, int M, bool REF>
+template <typename T, int M, bool REF, bool ROW>
 __device__ __forceinline__ void ysweep_tile(const lvc_tile tile, lvc_problem *probs,
                                             const double *__restrict__ geo,
                                             const T *__restrict__ Y,
@@ -629,14 +639,14 @@ __device__ __forceinline__ void ysweep_tile(const lvc_tile tile, lvc_problem *pr
 #pragma unroll
   for (int k = 0; k < M; ++k) s[k] = j0 + k < ny ? dt : T(1);
   SegSys<T, M> sys;
-  sys.prepare(a, c, s, make_lanes(q, G));
+  sys.template prepare<ROW>(a, c, s, make_lanes(q, G));
   for (int ci = 0; ci < cpl; ++ci) {
     const int col = ci * lc + lcol;  // lanes of a column interleave across cpl
     T d[M], x[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) d[k] = lds[lidx(j0 + k, col)];
-    sys.solve(d, x);
-    if (REF) refine(sys, d, a, c, s, x);
+    sys.template solve<ROW>(d, x);
+    if (REF) refine<ROW>(sys, d, a, c, s, x);
     // each lane overwrites only its own panel entries, read just above
 #pragma unroll
     for (int k = 0; k < M; ++k) lds[lidx(j0 + k, col)] = x[k];
@@ -661,8 +671,12 @@ __global__ __launch_bounds__(LVC_BLOCK) void lvc_ysweep_kernel(
     lvc_problem *probs, const lvc_tile *__restrict__ tiles,
     const double *__restrict__ geo, const T *__restrict__ Y, T *__restrict__ R) {
   extern __shared__ __align__(16) unsigned char dyn_lds[];
-  ysweep_tile<T, M, REF>(tiles[blockIdx.x], probs, geo, Y, R,
-                         reinterpret_cast<T *>(dyn_lds));
+  const lvc_tile tile = tiles[blockIdx.x];
+  T *lds = reinterpret_cast<T *>(dyn_lds);
+  if (tile.lanes <= 16)
+    ysweep_tile<T, M, REF, true>(tile, probs, geo, Y, R, lds);
+  else
+    ysweep_tile<T, M, REF, false>(tile, probs, geo, Y, R, lds);
 }
 
 // ---------------------------------------------------------------------------
