@@ -150,6 +150,14 @@ __device__ __forceinline__ T from_above(const Lanes &L, T v, int st, T ident) {
   return L.seg + st < L.G ? r : ident;
 }
 
+// Whether PCR level with stride st has work: st < G.  Groups exchanging
+// through ds_bpermute have at least 32 lanes, so their first five levels
+// always do (straight-line code, no branch).
+template <bool ROW>
+__device__ __forceinline__ bool level_used(const Lanes &L, int st) {
+  return (!ROW && st <= 16) || st < L.G;
+}
+
 // M consecutive values p[i0 .. i0+M-1] (0 past n); `vec` says p + i0 is 16-byte
 // aligned whenever the segment is complete.
 template <typename T, int M>
@@ -360,7 +368,7 @@ struct SegSys {
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
       const int st = 1 << l;
-      if (st < L.G) {
+      if (level_used<ROW>(L, st)) {
         const T Am = from_below<ROW>(L, A, st, T(0)), Bm = from_below<ROW>(L, B, st, T(1));
         const T Cm = from_below<ROW>(L, C, st, T(0));
         const T Ap = from_above<ROW>(L, A, st, T(0)), Bp = from_above<ROW>(L, B, st, T(1));
@@ -390,7 +398,7 @@ struct SegSys {
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) {
       const int st = 1 << l;
-      if (st < L.G) {
+      if (level_used<ROW>(L, st)) {
         const T Dm = from_below<ROW>(L, D, st, T(0)), Dp = from_above<ROW>(L, D, st, T(0));
         D = fma(-Dp, k2[l], fma(-Dm, k1[l], D));
       }
@@ -479,13 +487,28 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
   // occupancy (measured), so they load at the top of each iteration.
   constexpr bool kPrefetch = !REF;
   T rc[M], rm[M], rp[M], rl, rr;
+  // FP32 tiles: every lane loads from rows and columns inside the grid (the
+  // neighbours of boundary rows and columns clamped to the boundary) and the
+  // values outside are replaced by selects, not skipped by branches.  FP64
+  // tiles keep the branches (measured faster for them).
+  constexpr bool kClampedLoads = sizeof(T) == 4;
+  const int64_t dn = j > 0 ? -(int64_t)nx : 0, up = j < ny - 1 ? (int64_t)nx : 0;
+  const int il = min(max(i0 - 1, 0), nx - 1), ir = min(i0 + M, nx - 1);
   auto load_rows = [&](int b) {
     const T *row = R + probs[tile.problem + b].state_off + (int64_t)j * nx;
     load_seg<T, M>(row, i0, nx, vec, rc);
-    rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
-    rr = (i0 + M < nx) ? row[i0 + M] : T(0);
-    if (j > 0) load_seg<T, M>(row - nx, i0, nx, vec, rm);
-    if (j < ny - 1) load_seg<T, M>(row + nx, i0, nx, vec, rp);
+    if constexpr (kClampedLoads) {
+      load_seg<T, M>(row + dn, i0, nx, vec, rm);
+      load_seg<T, M>(row + up, i0, nx, vec, rp);
+      const T vl = row[il], vr = row[ir];
+      rl = (i0 > 0 && i0 - 1 < nx) ? vl : T(0);
+      rr = (i0 + M < nx) ? vr : T(0);
+    } else {
+      rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
+      rr = (i0 + M < nx) ? row[i0 + M] : T(0);
+      if (j > 0) load_seg<T, M>(row - nx, i0, nx, vec, rm);
+      if (j < ny - 1) load_seg<T, M>(row + nx, i0, nx, vec, rp);
+    }
   };
   T a[M], c[M], s[M];
   // U (the right-hand side) to registers; V (the explicit y term) and, for
