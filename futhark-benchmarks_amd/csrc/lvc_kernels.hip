@@ -208,6 +208,32 @@ __device__ __forceinline__ void store_seg(T *__restrict__ p, int i0, int n,
   }
 }
 
+// Buffer resource over one problem's [ny][nx] grid (raw, 32-bit elements
+// format word of the gfx9 family); loads and stores through it take a
+// per-lane byte offset and a wave-uniform one.
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const T *p, int nx, int ny) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), 0, nx * ny * (int)sizeof(T),
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, T v, int voff, int soff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+  } else {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+  }
+}
+
 // Coefficient tables of the geometry buffer in the sweep's real type.
 template <typename T>
 struct Taps {
@@ -936,60 +962,85 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   const int grp = wave / P, p = wave - grp * P;
   // Waves of column groups past the tile's columns leave before the first
   // barrier (a barrier waits only for the waves still running); the rest are
-  // threads 0 .. nact-1.
+  // threads 0 .. gused * P * 64 - 1.
   const int gused = min(groups, (tile.count + 63) / 64);
   if (grp >= gused) return;
-  const int nact = gused * P * 64;
   const int cidx = grp * 64 + lane;
   const bool active = cidx < tile.count;
   const int col = tile.first + cidx;
   const int r0 = p * M;
   const double *g = geo + pr.geo_off;
 
-  // This lane's right-hand side (the loads overlap the factor-table copy
-  // below).  Idle lanes read a valid column and discard it.
-  const bool full = r0 + M <= ny;  // wave-uniform
-  T d[M];
-  const T *src = Y + pr.state_off + (active ? col : tile.first) + (int64_t)r0 * nx;
-  if (full) {
-#pragma unroll
-    for (int k = 0; k < M; ++k) d[k] = src[(int64_t)k * nx];
-  } else {
-#pragma unroll
-    for (int k = 0; k < M; ++k) d[k] = r0 + k < ny ? src[(int64_t)k * nx] : T(0);
-  }
-  if (!active) {
-#pragma unroll
-    for (int k = 0; k < M; ++k) d[k] = T(0);
-  }
-
-  // The step's factor table (written by the x sweep) into LDS.
+  // The step's factor table (written by the x sweep), one row per thread
+  // (P * M <= 64 * P, no more than the threads still running), and the
+  // per-segment entries.  Every thread
+  // loads from valid addresses (clamped indexes, results discarded by
+  // selects), so that the code stays straight-line: these loads are issued
+  // before the column's, and the LDS writes and the barrier below wait for
+  // them only (loads complete in order).
   const T *f = yfac + (int64_t)pr.fac_src * LVC_YFAC_LEN;
-  for (int r = threadIdx.x; r < P * M; r += nact) {
-    L.m[r] = f[kFm * LVC_COL_ROWS + r];
-    L.rb[r] = f[kFrb * LVC_COL_ROWS + r];
-    L.crb[r] = f[kFcrb * LVC_COL_ROWS + r];
-    L.v[r] = f[kFv * LVC_COL_ROWS + r];
-    L.w[r] = f[kFw * LVC_COL_ROWS + r];
-  }
-  if (threadIdx.x < P) {
-    const T *fs = f + 6 * LVC_COL_ROWS;
-    L.aL[threadIdx.x] = fs[kSaL * kColWaves + threadIdx.x];
-    L.cL[threadIdx.x] = fs[kScL * kColWaves + threadIdx.x];
-    L.rm[threadIdx.x] = fs[kSrm * kColWaves + threadIdx.x];
-    L.rrb[threadIdx.x] = fs[kSrrb * kColWaves + threadIdx.x];
-    L.rC[threadIdx.x] = fs[kSrC * kColWaves + threadIdx.x];
-  }
+  const int fr = threadIdx.x;
+  const bool frow = fr < P * M;
+  const int fi = frow ? fr : 0;
+  const T fm = f[kFm * LVC_COL_ROWS + fi], frb = f[kFrb * LVC_COL_ROWS + fi];
+  const T fcrb = f[kFcrb * LVC_COL_ROWS + fi];
+  const T fv = f[kFv * LVC_COL_ROWS + fi], fw = f[kFw * LVC_COL_ROWS + fi];
+  const bool fseg = fr < P;
+  const T *fs = f + 6 * LVC_COL_ROWS + (fseg ? fr : 0);
+  const T saL = fs[kSaL * kColWaves], scL = fs[kScL * kColWaves];
+  const T srm = fs[kSrm * kColWaves], srrb = fs[kSrrb * kColWaves];
+  const T srC = fs[kSrC * kColWaves];
+  // The residual uses the same (T-rounded) taps as the explicit y term of
+  // the x sweep, so that the refined solve stays consistent with it.
+  T ta = T(0), tc = T(0);
   if (REF) {
-    // The residual uses the same (T-rounded) taps as the explicit y term of
-    // the x sweep, so that the refined solve stays consistent with it.
     const Taps<T> tp(g, nx, ny);
-    for (int r = threadIdx.x; r < P * M; r += nact) {
-      L.a64[r] = r < ny ? (double)tp.ay[r] : 0.0;
-      L.c64[r] = r < ny ? (double)tp.cy[r] : 0.0;
-    }
+    const int ti = fr < ny ? fr : 0;
+    ta = tp.ay[ti];
+    tc = tp.cy[ti];
   }
+  __builtin_amdgcn_sched_barrier(0);
+
+  // This lane's right-hand side, by buffer loads: the column is the lane's
+  // offset and each row a scalar offset (no per-row address arithmetic).
+  // Rows past the grid (a last, partial segment) read its last row here and
+  // are zeroed after the barrier, so that nothing waits for these loads
+  // before it.  Idle lanes read a valid column: their (finite) values stay
+  // in their own column, which is never stored.
+  const int lane_off = (active ? col : tile.first) * (int)sizeof(T);
+  const int row_bytes = nx * (int)sizeof(T);
+  const __amdgpu_buffer_rsrc_t in = grid_rsrc(Y + pr.state_off, nx, ny);
+  T d[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+    d[k] = buffer_load<T>(in, lane_off, min(r0 + k, ny - 1) * row_bytes);
+
+  // Every thread writes: threads past the table's rows (segments) write row
+  // (segment) 0 again, with the values they loaded for it.
+  const int fw_at = frow ? fr : 0, fs_at = fseg ? fr : 0;
+  L.m[fw_at] = fm;
+  L.rb[fw_at] = frb;
+  L.crb[fw_at] = fcrb;
+  L.v[fw_at] = fv;
+  L.w[fw_at] = fw;
+  if (REF) {
+    L.a64[fw_at] = fw_at < ny ? (double)ta : 0.0;
+    L.c64[fw_at] = fw_at < ny ? (double)tc : 0.0;
+  }
+  L.aL[fs_at] = saL;
+  L.cL[fs_at] = scL;
+  L.rm[fs_at] = srm;
+  L.rrb[fs_at] = srrb;
+  L.rC[fs_at] = srC;
+  // FP64 columns (twice the bytes) start after their loads have landed:
+  // measured faster than overlapping the forward sweep with them.
+  if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   __syncthreads();
+  if (r0 + M > ny) {  // wave-uniform
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      if (r0 + k >= ny) d[k] = T(0);
+  }
 
   const int gbase = grp * P;
   T x[M];
@@ -1017,15 +1068,15 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] += d[k];
   }
-  T *dst = R + pr.state_off + col + (int64_t)r0 * nx;
   if (active) {
-    if (full) {
+    const __amdgpu_buffer_rsrc_t out = grid_rsrc(R + pr.state_off, nx, ny);
+    if (r0 + M <= ny) {  // wave-uniform
 #pragma unroll
-      for (int k = 0; k < M; ++k) dst[(int64_t)k * nx] = x[k];
+      for (int k = 0; k < M; ++k) buffer_store<T>(out, x[k], lane_off, (r0 + k) * row_bytes);
     } else {
 #pragma unroll
       for (int k = 0; k < M; ++k)
-        if (r0 + k < ny) dst[(int64_t)k * nx] = x[k];
+        if (r0 + k < ny) buffer_store<T>(out, x[k], lane_off, (r0 + k) * row_bytes);
     }
   }
   // The last of the problem's workgroups to finish advances its time index
