@@ -129,39 +129,43 @@ __global__ __launch_bounds__(kThreads) void tridag_par_kernel(
 
 // Staged variant for systems of up to kStagedRows unknowns: one wavefront per
 // 64 systems.  Chunks of kChunk columns of a, b, c, y are loaded into LDS
-// with coalesced row reads (each system's row is contiguous) and read back
-// one system per lane.  c' (or the scan form's l1) waits in LDS; y' (or l0)
-// goes to the output array in place through the same transpose, and the
-// backward pass reads it back chunk by chunk in reverse.  The arithmetic and
-// its order per system are those of the kernels above (recurrences 1-3 of
-// the scan form advance together: each step uses only earlier steps' values,
-// so interleaving them changes no result).
+// with coalesced row reads (each system's row is contiguous; kChunk * 4 =
+// 128 bytes per system row) and read back one system per lane.  The forward
+// pass's per-row values for the backward pass go back through the same
+// transpose to global memory: y' (or l0) into the output array in place, c'
+// (or the scan form's l1) into a scratch array of the same shape, so that
+// the workgroup's LDS is the staging buffer alone and every workgroup of a
+// launch is resident at once.  The backward pass reads both back chunk by
+// chunk in reverse.  The arithmetic and its order per system are those of
+// the kernels above (recurrences 1-3 of the scan form advance together: each
+// step uses only earlier steps' values, so interleaving them changes no
+// result).
 constexpr int kStagedRows = 128;
-constexpr int kChunk = 16;
+constexpr int kChunk = 32;
 constexpr int kRowStep = kThreads / kChunk;  // rows per staging instruction
+constexpr int kQ = kThreads / kRowStep;      // staging instructions per array
 
 struct Stage {
-  float v[4][kThreads][kChunk + 1];  // a, b, c, y (y's slot: y' or l0 after use)
+  float v[4][kThreads][kChunk + 1];  // a, b, c, y; after use c' (l1) and y' (l0)
 };
 
 // The same rows and columns loaded into registers (this lane's slots), to be
 // written to LDS later: the next chunk's loads fly while this one computes.
-__device__ __forceinline__ void fetch(float (&reg)[kThreads / kRowStep],
-                                      const float *__restrict__ src, int64_t s0, int rows,
-                                      int m, int i0, int lane) {
+__device__ __forceinline__ void fetch(float (&reg)[kQ], const float *__restrict__ src,
+                                      int64_t s0, int rows, int m, int i0, int lane) {
   const int sr = lane / kChunk, sc = lane % kChunk, i = i0 + sc;
 #pragma unroll
-  for (int q = 0; q < kThreads / kRowStep; ++q) {
+  for (int q = 0; q < kQ; ++q) {
     const int r = q * kRowStep + sr;
     reg[q] = (r < rows && i < m) ? src[(s0 + r) * m + i] : 0.0f;
   }
 }
 
 __device__ __forceinline__ void deposit(float (&st)[kThreads][kChunk + 1],
-                                        const float (&reg)[kThreads / kRowStep], int lane) {
+                                        const float (&reg)[kQ], int lane) {
   const int sr = lane / kChunk, sc = lane % kChunk;
 #pragma unroll
-  for (int q = 0; q < kThreads / kRowStep; ++q) st[q * kRowStep + sr][sc] = reg[q];
+  for (int q = 0; q < kQ; ++q) st[q * kRowStep + sr][sc] = reg[q];
 }
 
 __device__ __forceinline__ void stage_out(const float (&st)[kThreads][kChunk + 1],
@@ -169,7 +173,7 @@ __device__ __forceinline__ void stage_out(const float (&st)[kThreads][kChunk + 1
                                           int m, int i0, int lane) {
   const int sr = lane / kChunk, sc = lane % kChunk, i = i0 + sc;
 #pragma unroll
-  for (int q = 0; q < kThreads / kRowStep; ++q) {
+  for (int q = 0; q < kQ; ++q) {
     const int r = q * kRowStep + sr;
     if (r < rows && i < m) dst[(s0 + r) * m + i] = st[r][sc];
   }
@@ -179,9 +183,8 @@ template <bool SEQ>
 __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
     const float *__restrict__ a, const float *__restrict__ b,
     const float *__restrict__ c, const float *__restrict__ y,
-    float *__restrict__ out, int64_t n, int m) {
+    float *__restrict__ out, float *__restrict__ cw, int64_t n, int m) {
   __shared__ Stage S;
-  __shared__ float fw[kStagedRows][kThreads];  // c' or l1, column = lane
   const int lane = threadIdx.x;
   const int64_t s0 = (int64_t)blockIdx.x * kThreads;
   const int rows = (int)min((int64_t)kThreads, n - s0);
@@ -192,7 +195,6 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
   float t0 = 1.0f, t1 = 0.0f, t2 = 0.0f, t3 = 1.0f, b0 = 0.0f, bprev = 0.0f;
   float f0 = 0.0f, f1 = 1.0f, y0 = 0.0f, last_bb = 1.0f, last_yy = 0.0f;
 
-  constexpr int kQ = kThreads / kRowStep;
   float pa[kQ], pb[kQ], pc[kQ], py[kQ];  // the next chunk, in flight
   fetch(pa, a, s0, rows, m, 0, lane);
   fetch(pb, b, s0, rows, m, 0, lane);
@@ -211,7 +213,6 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
       fetch(pc, c, s0, rows, m, i0 + kChunk, lane);
       fetch(py, y, s0, rows, m, i0 + kChunk, lane);
     }
-#pragma unroll
     for (int j = 0; j < kChunk; ++j) {
       const int i = i0 + j;
       if (i < m) {
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
             cprev = ci * norm;
             yprev = (yi - ai * yprev) * norm;
           }
-          fw[i][lane] = cprev;
+          S.v[2][lane][j] = cprev;
           S.v[3][lane][j] = yprev;
         } else {
           // recurrence 1 (c_{i-1} carried in cprev)
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
           }
           // this row's element of recurrence 3
           S.v[3][lane][j] = yy / bb;
-          fw[i][lane] = 0.0f - ci / bb;
+          S.v[2][lane][j] = 0.0f - ci / bb;
           bprev = bb;
           cprev = ci;
           last_bb = bb;
@@ -268,6 +269,7 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
       }
     }
     __syncthreads();
+    stage_out(S.v[2], cw, s0, rows, m, i0, lane);
     stage_out(S.v[3], out, s0, rows, m, i0, lane);
     __syncthreads();
   }
@@ -276,23 +278,28 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
   float xnext = yprev;                                  // Thomas: x_{m-1} = y'_{m-1}
   const float yn = SEQ ? 0.0f : last_yy / last_bb;      // scan form
   float g0 = 0.0f, g1 = 1.0f;
+  fetch(pc, cw, s0, rows, m, (nch - 1) * kChunk, lane);
   fetch(py, out, s0, rows, m, (nch - 1) * kChunk, lane);
   for (int ch = nch - 1; ch >= 0; --ch) {
     const int i0 = ch * kChunk;
+    deposit(S.v[2], pc, lane);
     deposit(S.v[3], py, lane);
     __syncthreads();
-    if (ch > 0) fetch(py, out, s0, rows, m, i0 - kChunk, lane);
-#pragma unroll
+    if (ch > 0) {
+      fetch(pc, cw, s0, rows, m, i0 - kChunk, lane);
+      fetch(py, out, s0, rows, m, i0 - kChunk, lane);
+    }
     for (int j = kChunk - 1; j >= 0; --j) {
       const int i = i0 + j;
       if (i < m) {
+        const float fwi = S.v[2][lane][j];
         if (i == m - 1) {
           S.v[3][lane][j] = SEQ ? yprev : 0.0f + 1.0f * yn;
         } else if (SEQ) {
-          xnext = S.v[3][lane][j] - fw[i][lane] * xnext;
+          xnext = S.v[3][lane][j] - fwi * xnext;
           S.v[3][lane][j] = xnext;
         } else {
-          const float h0 = S.v[3][lane][j] + fw[i][lane] * g0, h1 = g1 * fw[i][lane];
+          const float h0 = S.v[3][lane][j] + fwi * g0, h1 = g1 * fwi;
           g0 = h0;
           g1 = h1;
           S.v[3][lane][j] = g0 + g1 * yn;
@@ -310,7 +317,8 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
 extern "C" {
 
 // form: 0 = tridagPar (scan form), 1 = tridagSeq (Thomas).  scratch: n*m
-// floats, used only when m > the LDS limit (may be NULL otherwise).
+// floats, used when m <= lvc_tridag_staged_rows() (the staged kernel's c')
+// or m > lvc_tridag_lds_rows(); may be NULL otherwise.
 int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
                       const float *c, const float *y, float *out, float *scratch,
                       int64_t n, int m) {
@@ -321,10 +329,11 @@ int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
   if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
   if (form != 0 && form != 1) return (int)hipErrorInvalidValue;
   if (m <= kStagedRows) {
+    if (!scratch) return (int)hipErrorInvalidValue;
     if (form == 1)
-      tridag_staged_kernel<true><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, n, m);
+      tridag_staged_kernel<true><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, scratch, n, m);
     else
-      tridag_staged_kernel<false><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, n, m);
+      tridag_staged_kernel<false><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, scratch, n, m);
     return (int)hipGetLastError();
   }
   const bool lds = m <= kLdsRows;
@@ -347,5 +356,6 @@ int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
 }
 
 int lvc_tridag_lds_rows(void) { return kLdsRows; }
+int lvc_tridag_staged_rows(void) { return kStagedRows; }
 
 }  // extern "C"
