@@ -213,6 +213,7 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
       fetch(pc, c, s0, rows, m, i0 + kChunk, lane);
       fetch(py, y, s0, rows, m, i0 + kChunk, lane);
     }
+#pragma unroll
     for (int j = 0; j < kChunk; ++j) {
       const int i = i0 + j;
       if (i < m) {
@@ -289,6 +290,7 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
       fetch(pc, cw, s0, rows, m, i0 - kChunk, lane);
       fetch(py, out, s0, rows, m, i0 - kChunk, lane);
     }
+#pragma unroll
     for (int j = kChunk - 1; j >= 0; --j) {
       const int i = i0 + j;
       if (i < m) {
