@@ -881,30 +881,27 @@ __device__ void ycol_factor(const lvc_problem &pr, const double *g, T dt, int M,
 template <typename T, int M>
 struct ColLds {
   static constexpr int kRows = kColWaves * M;
-  T m[kRows], rb[kRows], crb[kRows], v[kRows], w[kRows];  // per row
-  double a64[kRows], c64[kRows];  // the taps in FP64 for the residual
-  T aL[kColWaves], cL[kColWaves];                  // each segment's last row
-  T rm[kColWaves], rrb[kColWaves], rC[kColWaves];  // reduced system's factor
-  T y0[2][LVC_COL_BLOCK], Dt[2][LVC_COL_BLOCK];    // exchange, one per solve
+  double a64[kRows], c64[kRows];                // the taps in FP64 for the residual
+  T y0[2][LVC_COL_BLOCK], Dt[2][LVC_COL_BLOCK];  // exchange, one per solve
 };
 
 // Solves in place for the right-hand side held in x (this lane's segment p of
 // one column), using exchange slot `sl`.  Returns the neighbouring unknowns
 // XL = x_{-1} (previous segment's last) and XN = x_M (next segment's first).
 template <typename T, int M>
-__device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int P,
+__device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *__restrict__ F, int p, int P,
                                           int gbase, int lane, int sl, T &XL, T &XN) {
   const int r0 = p * M;
   const T dlast = x[M - 1];
 #pragma unroll
-  for (int k = 1; k < M - 1; ++k) x[k] = fma(-L.m[r0 + k], x[k - 1], x[k]);
-  x[M - 2] *= L.rb[r0 + M - 2];
+  for (int k = 1; k < M - 1; ++k) x[k] = fma(-F[kFm * LVC_COL_ROWS + r0 + k], x[k - 1], x[k]);
+  x[M - 2] *= F[kFrb * LVC_COL_ROWS + r0 + M - 2];
 #pragma unroll
   for (int k = M - 3; k >= 0; --k)
-    x[k] = fma(-L.crb[r0 + k], x[k + 1], x[k] * L.rb[r0 + k]);
+    x[k] = fma(-F[kFcrb * LVC_COL_ROWS + r0 + k], x[k + 1], x[k] * F[kFrb * LVC_COL_ROWS + r0 + k]);
   const int me = (gbase + p) * 64 + lane;
   L.y0[sl][me] = x[0];
-  L.Dt[sl][me] = fma(-L.aL[p], x[M - 2], dlast);
+  L.Dt[sl][me] = fma(-F[6 * LVC_COL_ROWS + kSaL * kColWaves + p], x[M - 2], dlast);
   __syncthreads();
   // Reduced system of the column: Thomas over its P segments (slots
   // gbase .. gbase+P-1, all below kColWaves).
@@ -916,8 +913,8 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
       const int at = (gbase + q) * 64 + lane;
       const T yn = q + 1 < P ? L.y0[sl][at + 64] : T(0);
       if (q == p) y0n = yn;
-      const T D = fma(-L.cL[q], yn, L.Dt[sl][at]);
-      prev = q == 0 ? D : fma(-L.rm[q], prev, D);
+      const T D = fma(-F[6 * LVC_COL_ROWS + kScL * kColWaves + q], yn, L.Dt[sl][at]);
+      prev = q == 0 ? D : fma(-F[6 * LVC_COL_ROWS + kSrm * kColWaves + q], prev, D);
       Yr[q] = prev;
     }
   }
@@ -925,7 +922,7 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
 #pragma unroll
   for (int q = kColWaves - 1; q >= 0; --q) {
     if (q < P) {
-      nxt = fma(-L.rC[q], nxt, Yr[q]) * L.rrb[q];
+      nxt = fma(-F[6 * LVC_COL_ROWS + kSrC * kColWaves + q], nxt, Yr[q]) * F[6 * LVC_COL_ROWS + kSrrb * kColWaves + q];
       if (q == p + 1) Xp = nxt;
       if (q == p) Xc = nxt;
       if (q == p - 1) Xm = nxt;
@@ -936,10 +933,10 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, int p, int
   // segment holds (the refinement's residual needs the exact neighbour).
 #pragma unroll
   for (int k = 0; k < M - 1; ++k)
-    x[k] = fma(L.w[r0 + k], Xc, fma(L.v[r0 + k], Xm, x[k]));
+    x[k] = fma(F[kFw * LVC_COL_ROWS + r0 + k], Xc, fma(F[kFv * LVC_COL_ROWS + r0 + k], Xm, x[k]));
   x[M - 1] = Xc;
   XL = Xm;
-  XN = p + 1 < P ? fma(L.w[r0 + M], Xp, fma(L.v[r0 + M], Xc, y0n)) : T(0);
+  XN = p + 1 < P ? fma(F[kFw * LVC_COL_ROWS + r0 + M], Xp, fma(F[kFv * LVC_COL_ROWS + r0 + M], Xc, y0n)) : T(0);
 }
 
 // Segments of 32 rows: at least 4 waves per SIMD (two workgroups per CU), so
@@ -971,42 +968,47 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   const int r0 = p * M;
   const double *g = geo + pr.geo_off;
 
-  // The step's factor table (written by the x sweep), one row per thread
-  // (P * M <= 64 * P, no more than the threads still running), and the
-  // per-segment entries.  Every thread
-  // loads from valid addresses (clamped indexes, results discarded by
-  // selects), so that the code stays straight-line: these loads are issued
-  // before the column's, and the LDS writes and the barrier below wait for
-  // them only (loads complete in order).
-  const T *f = yfac + (int64_t)pr.fac_src * LVC_YFAC_LEN;
+  // The step's factor table (written by the x sweep).  Every lane of a wave
+  // works on the same rows, so the solve reads it at wave-uniform addresses:
+  // with the refinement straight from global memory (scalar loads into
+  // SGPRs, operands of the vector arithmetic directly, which keeps VGPRs for
+  // the refinement), without it from a copy in LDS in the same layout
+  // (measured faster there).  The copy takes one row per thread (P * M <=
+  // 64 * P, no more than the threads still running) and loads from valid
+  // addresses only (clamped indexes; threads past the rows write row 0
+  // again), so it stays straight-line and its loads are issued before the
+  // column's: the LDS writes and the barrier wait for them alone (loads
+  // complete in order).
+  const T *__restrict__ f = yfac + (int64_t)pr.fac_src * LVC_YFAC_LEN;
+  __shared__ T ftab[REF ? 1 : LVC_YFAC_LEN];
   const int fr = threadIdx.x;
-  const bool frow = fr < P * M;
-  const int fi = frow ? fr : 0;
-  const T fm = f[kFm * LVC_COL_ROWS + fi], frb = f[kFrb * LVC_COL_ROWS + fi];
-  const T fcrb = f[kFcrb * LVC_COL_ROWS + fi];
-  const T fv = f[kFv * LVC_COL_ROWS + fi], fw = f[kFw * LVC_COL_ROWS + fi];
-  const bool fseg = fr < P;
-  const T *fs = f + 6 * LVC_COL_ROWS + (fseg ? fr : 0);
-  const T saL = fs[kSaL * kColWaves], scL = fs[kScL * kColWaves];
-  const T srm = fs[kSrm * kColWaves], srrb = fs[kSrrb * kColWaves];
-  const T srC = fs[kSrC * kColWaves];
+  const int fi = fr < P * M ? fr : 0, si = fr < P ? fr : 0;
+  constexpr int kRowFields = kFw + 1, kSegFields = kSrC + 1;  // the fields the solve reads
+  T fv[kRowFields] = {}, sv[kSegFields] = {};
+  if (!REF) {
+#pragma unroll
+    for (int q = 0; q < kRowFields; ++q) fv[q] = f[q * LVC_COL_ROWS + fi];
+#pragma unroll
+    for (int q = 0; q < kSegFields; ++q) sv[q] = f[6 * LVC_COL_ROWS + q * kColWaves + si];
+  }
   // The residual uses the same (T-rounded) taps as the explicit y term of
-  // the x sweep, so that the refined solve stays consistent with it.
-  T ta = T(0), tc = T(0);
+  // the x sweep, so that the refined solve stays consistent with it: in LDS
+  // as FP64, one row per thread, ordered before their first use by the
+  // solve's barrier.
   if (REF) {
     const Taps<T> tp(g, nx, ny);
-    const int ti = fr < ny ? fr : 0;
-    ta = tp.ay[ti];
-    tc = tp.cy[ti];
+    if (fr < P * M) {
+      L.a64[fr] = fr < ny ? (double)tp.ay[fr] : 0.0;
+      L.c64[fr] = fr < ny ? (double)tp.cy[fr] : 0.0;
+    }
   }
   __builtin_amdgcn_sched_barrier(0);
 
   // This lane's right-hand side, by buffer loads: the column is the lane's
   // offset and each row a scalar offset (no per-row address arithmetic).
-  // Rows past the grid (a last, partial segment) read its last row here and
-  // are zeroed after the barrier, so that nothing waits for these loads
-  // before it.  Idle lanes read a valid column: their (finite) values stay
-  // in their own column, which is never stored.
+  // Rows past the grid (a last, partial segment: wave-uniform) read its last
+  // row and are replaced by zeros.  Idle lanes read a valid column: their
+  // (finite) values stay in their own column, which is never stored.
   const int lane_off = (active ? col : tile.first) * (int)sizeof(T);
   const int row_bytes = nx * (int)sizeof(T);
   const __amdgpu_buffer_rsrc_t in = grid_rsrc(Y + pr.state_off, nx, ny);
@@ -1014,28 +1016,17 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
 #pragma unroll
   for (int k = 0; k < M; ++k)
     d[k] = buffer_load<T>(in, lane_off, min(r0 + k, ny - 1) * row_bytes);
-
-  // Every thread writes: threads past the table's rows (segments) write row
-  // (segment) 0 again, with the values they loaded for it.
-  const int fw_at = frow ? fr : 0, fs_at = fseg ? fr : 0;
-  L.m[fw_at] = fm;
-  L.rb[fw_at] = frb;
-  L.crb[fw_at] = fcrb;
-  L.v[fw_at] = fv;
-  L.w[fw_at] = fw;
-  if (REF) {
-    L.a64[fw_at] = fw_at < ny ? (double)ta : 0.0;
-    L.c64[fw_at] = fw_at < ny ? (double)tc : 0.0;
+  if (!REF) {
+#pragma unroll
+    for (int q = 0; q < kRowFields; ++q) ftab[q * LVC_COL_ROWS + fi] = fv[q];
+#pragma unroll
+    for (int q = 0; q < kSegFields; ++q) ftab[6 * LVC_COL_ROWS + q * kColWaves + si] = sv[q];
+    // FP64 columns (twice the bytes) start after their loads have landed:
+    // measured faster than overlapping the forward sweep with them.
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
   }
-  L.aL[fs_at] = saL;
-  L.cL[fs_at] = scL;
-  L.rm[fs_at] = srm;
-  L.rrb[fs_at] = srrb;
-  L.rC[fs_at] = srC;
-  // FP64 columns (twice the bytes) start after their loads have landed:
-  // measured faster than overlapping the forward sweep with them.
-  if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-  __syncthreads();
+  const T *F = REF ? f : ftab;
   if (r0 + M > ny) {  // wave-uniform
 #pragma unroll
     for (int k = 0; k < M; ++k)
@@ -1047,7 +1038,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
 #pragma unroll
   for (int k = 0; k < M; ++k) x[k] = d[k];
   T XL, XN;
-  col_solve<T, M>(x, L, p, P, gbase, lane, 0, XL, XN);
+  col_solve<T, M>(x, L, F, p, P, gbase, lane, 0, XL, XN);
   if (REF) {
     // Residual in FP64 (T-rounded taps, as above), solved again with the same
     // factorisation; the correction is added in T (one rounding of the
@@ -1064,7 +1055,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
       d[k] = (T)((double)d[k] - ax);
     }
     T EL, EN;
-    col_solve<T, M>(d, L, p, P, gbase, lane, 1, EL, EN);
+    col_solve<T, M>(d, L, F, p, P, gbase, lane, 1, EL, EN);
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] += d[k];
   }
