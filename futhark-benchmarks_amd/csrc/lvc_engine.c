@@ -81,13 +81,35 @@ static void nt_sort_desc(int *order, int n, const lvc_problem *probs) {
   }
 }
 
-/* Problems per x tile sharing a factorisation: lvc.xshare when set, else
- * the measured best at 128 x 256x256x64 (FP32+refinement and FP32: 8, 38.0K
- * and 58.8K options/s vs 31.5K and 43.0K unshared; FP64: 16, 36.8K vs 23.0K). */
+/* Problems per x tile sharing a factorisation when lvc.xshare is set (0
+ * when it is not: the batch then chooses, below). */
 int lvc_xshare(struct futhark_context *ctx, int precision) {
-  if (ctx->cfg && ctx->cfg->xshare > 0)
-    return ctx->cfg->xshare;
-  return precision == LVC_FP64 ? 16 : 8;
+  (void)precision;
+  return ctx->cfg && ctx->cfg->xshare > 0 ? ctx->cfg->xshare : 0;
+}
+
+/* The batch's default: FP64 shares 16 (its row factorisation is the dearest
+ * part; at 128 x 256x256x64 36.8K options/s against 23.0K unshared).  The
+ * FP32 forms share as few as still let every x workgroup of a step (each
+ * group's row tiles and its y-factor workgroup) be resident at once, three
+ * per CU (the shared kernels' occupancy), but at least 2: more groups are
+ * more waves to hide latency with (at 128 x 256x256x64, FP32+refinement:
+ * 6 per tile 46.9K options/s, 8 44.0K, 5 40.4K -- a second round of
+ * workgroups).  runs[r]: lengths of the runs of problems that could share;
+ * tiles[r]: x tiles per group of run r. */
+static int batch_xshare(struct futhark_context *ctx, int precision, int nruns,
+                        const int *runs, const int *tiles) {
+  if (precision == LVC_FP64)
+    return 16;
+  const int64_t slots = 3 * (int64_t)(ctx->num_cus > 0 ? ctx->num_cus : 256);
+  for (int x = 2; x < LVC_XSHARE_MAX; x++) {
+    int64_t wgs = 0;
+    for (int r = 0; r < nruns; r++)
+      wgs += (int64_t)((runs[r] + x - 1) / x) * (tiles[r] + 1);
+    if (wgs <= slots)
+      return x;
+  }
+  return LVC_XSHARE_MAX;
 }
 
 /* Problems whose factorisations coincide: everything but the strike (and
@@ -109,7 +131,7 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   if (n == 0)
     return 0;
   const size_t real = precision == LVC_FP64 ? sizeof(double) : sizeof(float);
-  const int xshare = lvc_xshare(ctx, precision);
+  int xshare = lvc_xshare(ctx, precision);
   b->h_probs = malloc((size_t)n * sizeof(lvc_problem));
   int *order = malloc((size_t)n * sizeof(int));
   int *gsize = malloc((size_t)n * sizeof(int)); /* by position in order */
@@ -150,6 +172,23 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
   for (int p = 0; p < n; p++)
     order[p] = p;
   nt_sort_desc(order, n, b->h_probs);
+  if (xshare <= 0) {
+    /* the runs of consecutive problems that could share, unbounded */
+    int nruns = 0;
+    for (int oi = 0; oi < n;) {
+      const int p = order[oi];
+      int cnt = 1;
+      while (oi + cnt < n && order[oi + cnt] == p + cnt &&
+             same_factor(&b->h_probs[p], &b->h_probs[p + cnt]))
+        cnt++;
+      const int rows = LVC_BLOCK / lvc_lanes_for(b->h_probs[p].nx, seg_x);
+      gsize[nruns] = cnt;                                    /* run length */
+      leaders[nruns] = (b->h_probs[p].ny + rows - 1) / rows; /* tiles per group */
+      nruns++;
+      oi += cnt;
+    }
+    xshare = batch_xshare(ctx, precision, nruns, gsize, leaders);
+  }
   for (int oi = 0; oi < n;) {
     const int p = order[oi];
     int cnt = 1;
