@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../../include/locvolcalib.h"
 #include "lvc_runtime.h"
 
 void *lvc_array_mem_f64(const struct futhark_f64_1d *a);

@@ -9,7 +9,7 @@
 
 #include <hip/hip_runtime_api.h>
 
-#include "../../include/locvolcalib.h"
+#include "../../include/futhark_context.h"
 #include "lvc_internal.h"
 
 #define LVC_MAX_TUNED 64

@@ -23,6 +23,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../../include/locvolcalib.h"
 #include "lvc_runtime.h"
 
 #define SLOT_MAX_N 512 /* largest numX / numY a server slot holds */

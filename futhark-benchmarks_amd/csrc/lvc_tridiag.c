@@ -9,120 +9,10 @@
 #include <string.h>
 
 #include "../../include/ocean_tridiag.h"
+#include "lvc_ndarray.h"
 #include "lvc_runtime.h"
 
 #define INNER_DIM 115 /* tridiag-test.fut:4 */
-
-struct futhark_f32_2d {
-  void *mem;
-  int64_t shape[2];
-  size_t bytes;
-  int raw; /* wraps caller memory: not freed */
-};
-
-static int dims_ok(struct futhark_context *ctx, int64_t d0, int64_t d1) {
-  if (d0 < 0 || d1 < 0) {
-    lvc_set_error(ctx, "negative array size %lld x %lld", (long long)d0,
-                  (long long)d1);
-    return 0;
-  }
-  if (d1 != 0 && d0 > (int64_t)(SIZE_MAX / sizeof(float)) / d1) {
-    lvc_set_error(ctx, "array of %lld x %lld floats is too large",
-                  (long long)d0, (long long)d1);
-    return 0;
-  }
-  return 1;
-}
-
-struct futhark_f32_2d *futhark_new_f32_2d(struct futhark_context *ctx,
-                                          const float *data, int64_t dim0,
-                                          int64_t dim1) {
-  if (!lvc_ready(ctx) || !dims_ok(ctx, dim0, dim1))
-    return NULL;
-  struct futhark_f32_2d *arr = calloc(1, sizeof(*arr));
-  if (!arr)
-    return NULL;
-  arr->shape[0] = dim0;
-  arr->shape[1] = dim1;
-  arr->bytes = (size_t)dim0 * (size_t)dim1 * sizeof(float);
-  pthread_mutex_lock(&ctx->lock);
-  int rc = lvc_dev_alloc(ctx, arr->bytes, &arr->mem);
-  if (!rc && data && arr->bytes)
-    rc = lvc_check_hip(ctx, hipMemcpyAsync(arr->mem, data, arr->bytes,
-                                           hipMemcpyHostToDevice, ctx->stream),
-                       "hipMemcpyAsync") ||
-         lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream),
-                       "hipStreamSynchronize");
-  if (rc && arr->mem)
-    lvc_dev_free(ctx, arr->mem, arr->bytes);
-  pthread_mutex_unlock(&ctx->lock);
-  if (rc) {
-    free(arr);
-    return NULL;
-  }
-  return arr;
-}
-
-struct futhark_f32_2d *futhark_new_raw_f32_2d(struct futhark_context *ctx,
-                                              void *data, int64_t dim0,
-                                              int64_t dim1) {
-  if (!lvc_ready(ctx) || !dims_ok(ctx, dim0, dim1))
-    return NULL;
-  struct futhark_f32_2d *arr = calloc(1, sizeof(*arr));
-  if (!arr)
-    return NULL;
-  arr->mem = data;
-  arr->shape[0] = dim0;
-  arr->shape[1] = dim1;
-  arr->bytes = (size_t)dim0 * (size_t)dim1 * sizeof(float);
-  arr->raw = 1;
-  return arr;
-}
-
-int futhark_free_f32_2d(struct futhark_context *ctx, struct futhark_f32_2d *arr) {
-  if (!ctx || !arr)
-    return 0;
-  pthread_mutex_lock(&ctx->lock);
-  if (!arr->raw && ctx->ok)
-    lvc_dev_free(ctx, arr->mem, arr->bytes);
-  pthread_mutex_unlock(&ctx->lock);
-  free(arr);
-  return 0;
-}
-
-int futhark_values_f32_2d(struct futhark_context *ctx, struct futhark_f32_2d *arr,
-                          float *data) {
-  if (!lvc_ready(ctx))
-    return 1;
-  if (!arr) {
-    lvc_set_error(ctx, "NULL array");
-    return 1;
-  }
-  if (arr->bytes == 0)
-    return 0;
-  pthread_mutex_lock(&ctx->lock);
-  int rc = lvc_check_hip(ctx, hipMemcpyAsync(data, arr->mem, arr->bytes,
-                                             hipMemcpyDeviceToHost, ctx->stream),
-                         "hipMemcpyAsync") ||
-           lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream),
-                         "hipStreamSynchronize");
-  if (!rc)
-    lvc_prof_collect(ctx);
-  pthread_mutex_unlock(&ctx->lock);
-  return rc;
-}
-
-void *futhark_values_raw_f32_2d(struct futhark_context *ctx,
-                                struct futhark_f32_2d *arr) {
-  (void)ctx;
-  return arr ? arr->mem : NULL;
-}
-
-const int64_t *futhark_shape_f32_2d(struct futhark_context *ctx,
-                                    struct futhark_f32_2d *arr) {
-  (void)ctx;
-  return arr ? arr->shape : NULL;
-}
 
 /* form 0: tridagPar, 1: tridagSeq; inner: required m (0 = any). */
 static int tridag_entry(struct futhark_context *ctx, struct futhark_f32_2d **out0,
@@ -140,13 +30,13 @@ static int tridag_entry(struct futhark_context *ctx, struct futhark_f32_2d **out
     lvc_set_error(ctx, "%s: NULL input array", name);
     return 1;
   }
-  const int64_t n = a->shape[0], m = a->shape[1];
+  const int64_t n = a->a.shape[0], m = a->a.shape[1];
   const struct futhark_f32_2d *in[3] = {b, c, y};
   for (int k = 0; k < 3; k++)
-    if (in[k]->shape[0] != n || in[k]->shape[1] != m) {
+    if (in[k]->a.shape[0] != n || in[k]->a.shape[1] != m) {
       lvc_set_error(ctx, "%s: input %d has shape [%lld][%lld], input 0 [%lld][%lld]",
-                    name, k + 1, (long long)in[k]->shape[0],
-                    (long long)in[k]->shape[1], (long long)n, (long long)m);
+                    name, k + 1, (long long)in[k]->a.shape[0],
+                    (long long)in[k]->a.shape[1], (long long)n, (long long)m);
       return 1;
     }
   if (inner && m != inner) {
@@ -168,26 +58,27 @@ static int tridag_entry(struct futhark_context *ctx, struct futhark_f32_2d **out
     lvc_set_error(ctx, "host allocation failed");
     return 1;
   }
-  out->shape[0] = n;
-  out->shape[1] = m;
-  out->bytes = (size_t)n * (size_t)m * sizeof(float);
+  out->a.shape[0] = n;
+  out->a.shape[1] = m;
+  out->a.shape[2] = 1;
+  out->a.bytes = (size_t)n * (size_t)m * sizeof(float);
   pthread_mutex_lock(&ctx->lock);
   void *scratch = NULL;
   const size_t sbytes =
-      (m <= lvc_tridag_staged_rows() || m > lvc_tridag_lds_rows()) ? out->bytes : 0;
-  int rc = lvc_dev_alloc(ctx, out->bytes, &out->mem) ||
+      (m <= lvc_tridag_staged_rows() || m > lvc_tridag_lds_rows()) ? out->a.bytes : 0;
+  int rc = lvc_dev_alloc(ctx, out->a.bytes, &out->a.mem) ||
            (sbytes && lvc_dev_alloc(ctx, sbytes, &scratch));
   if (!rc && n > 0) {
     int tok = lvc_prof_begin(ctx, "lvc_tridag");
-    int e = lvc_launch_tridag((void *)ctx->stream, form, a->mem, b->mem, c->mem,
-                              y->mem, out->mem, scratch, n, (int)m);
+    int e = lvc_launch_tridag((void *)ctx->stream, form, a->a.mem, b->a.mem, c->a.mem,
+                              y->a.mem, out->a.mem, scratch, n, (int)m);
     lvc_prof_end(ctx, tok);
     rc = lvc_check_hip(ctx, e, "tridiagonal kernel");
   }
   if (scratch)
     lvc_dev_free(ctx, scratch, sbytes);
-  if (rc && out->mem)
-    lvc_dev_free(ctx, out->mem, out->bytes);
+  if (rc && out->a.mem)
+    lvc_dev_free(ctx, out->a.mem, out->a.bytes);
   pthread_mutex_unlock(&ctx->lock);
   if (rc) {
     free(out);

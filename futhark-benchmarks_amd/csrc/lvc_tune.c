@@ -18,6 +18,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../../include/locvolcalib.h"
 #include "lvc_runtime.h"
 
 static int table_find(struct futhark_context *ctx, int precision, int nx,

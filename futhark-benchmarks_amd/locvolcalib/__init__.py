@@ -101,6 +101,18 @@ _SIGNATURES = {
     "futhark_values_f32_2d": (ctypes.c_int, [_vp, _vp, _vp]),
     "futhark_values_raw_f32_2d": (_vp, [_vp, _vp]),
     "futhark_shape_f32_2d": (ctypes.POINTER(_i64), [_vp, _vp]),
+    "futhark_new_i32_2d": (_vp, [_vp, _vp, _i64, _i64]),
+    "futhark_new_raw_i32_2d": (_vp, [_vp, _vp, _i64, _i64]),
+    "futhark_free_i32_2d": (ctypes.c_int, [_vp, _vp]),
+    "futhark_values_i32_2d": (ctypes.c_int, [_vp, _vp, _vp]),
+    "futhark_values_raw_i32_2d": (_vp, [_vp, _vp]),
+    "futhark_shape_i32_2d": (ctypes.POINTER(_i64), [_vp, _vp]),
+    "futhark_new_f32_3d": (_vp, [_vp, _vp, _i64, _i64, _i64]),
+    "futhark_new_raw_f32_3d": (_vp, [_vp, _vp, _i64, _i64, _i64]),
+    "futhark_free_f32_3d": (ctypes.c_int, [_vp, _vp]),
+    "futhark_values_f32_3d": (ctypes.c_int, [_vp, _vp, _vp]),
+    "futhark_values_raw_f32_3d": (_vp, [_vp, _vp]),
+    "futhark_shape_f32_3d": (ctypes.POINTER(_i64), [_vp, _vp]),
     "futhark_entry_tridagNested": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),
     "futhark_entry_tridagNestedConst": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),
     "futhark_entry_tridagNestedSeq": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 4),

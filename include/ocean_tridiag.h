@@ -25,28 +25,14 @@
 
 #include <stdint.h>
 
+#include "futhark_arrays.h"
 #include "locvolcalib.h"
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 
-struct futhark_f32_2d;
-
-/* [dim0][dim1] row-major f32 arrays (the generated API's 2-D arrays). */
-struct futhark_f32_2d *futhark_new_f32_2d(struct futhark_context *ctx,
-                                          const float *data, int64_t dim0,
-                                          int64_t dim1);
-struct futhark_f32_2d *futhark_new_raw_f32_2d(struct futhark_context *ctx,
-                                              void *data, int64_t dim0,
-                                              int64_t dim1);
-int futhark_free_f32_2d(struct futhark_context *ctx, struct futhark_f32_2d *arr);
-int futhark_values_f32_2d(struct futhark_context *ctx, struct futhark_f32_2d *arr,
-                          float *data);
-void *futhark_values_raw_f32_2d(struct futhark_context *ctx,
-                                struct futhark_f32_2d *arr);
-const int64_t *futhark_shape_f32_2d(struct futhark_context *ctx,
-                                    struct futhark_f32_2d *arr);
+/* The [n][m] f32 arrays are those of futhark_arrays.h. */
 
 /* tridiag-test.fut:14-15 */
 int futhark_entry_tridagNested(struct futhark_context *ctx,

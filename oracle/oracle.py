@@ -240,3 +240,78 @@ def explicit_method(D, DD, mu, var, res) -> np.ndarray:
     getattr(lib(), "lvc_explicit" + suf)(_p(D), _p(DD), _p(mu), _p(var), _p(res),
                                          rows, m, _p(out))
     return out
+
+
+# ---- OptionPricing (finpar/OptionPricing.fut) -------------------------------
+# op_oracle.c: the per-path payoffs of the reference in f32 (Sobol :12-35,
+# inverse Gaussian :40-140, Brownian bridge :145-183, Black-Scholes
+# :190-225, payoffs :230-293), summed over paths in f64.  Pinned by the
+# reference's three datasets (OptionPricing-data/{small,medium,large}.out),
+# the Sobol construction against a brute-force Gray-code recurrence, and the
+# inverse Gaussian against the normal CDF (tests/test_oracle_op.py).
+
+def op_sobol(n: int, dir_vs: np.ndarray) -> np.ndarray:
+    """sobolIndR (:29-35) of path n over dir_vs [dims][bits]."""
+    dv = np.ascontiguousarray(dir_vs, dtype=np.int32)
+    out = np.empty(dv.shape[0], np.float32)
+    L = lib()
+    L.op_sobol.argtypes = [_c_i32, _ptr, _c_i64, _c_i64, _ptr]
+    L.op_sobol(n, _p(dv), dv.shape[0], dv.shape[1], _p(out))
+    return out
+
+
+def op_ugaussian(p: float) -> float:
+    """ugaussianEl (:121-136) in f32."""
+    L = lib()
+    L.op_ugaussian.argtypes = [ctypes.c_float]
+    L.op_ugaussian.restype = ctypes.c_float
+    return float(L.op_ugaussian(ctypes.c_float(p)))
+
+
+def op_main(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, md_detvals,
+            md_discts, bb_inds, bb_data, paths=None, nthreads=None) -> np.ndarray:
+    """main (:295-322) over the first `paths` paths (default num_mc_it): the
+    per-model mean payoff, the f64 sum of the f32 payoffs / paths."""
+    n = int(num_mc_it if paths is None else paths)
+    dv = np.ascontiguousarray(dir_vs, dtype=np.int32)
+    arrs = [np.ascontiguousarray(a, dtype=np.float32)
+            for a in (md_cs, md_vols, md_drifts, md_sts, md_detvals, md_discts, bb_data)]
+    cs, vols, drifts, sts, detv, disc, bbd = arrs
+    bbi = np.ascontiguousarray(bb_inds, dtype=np.int32)
+    M, U = sts.shape
+    D = bbi.shape[1]
+    K = disc.shape[1]
+    sums = np.zeros(M, np.float64)
+    L = lib()
+    L.op_sum_payoffs.argtypes = [_c_i32, _c_i64, _c_i64, _ptr, _c_i64] + [_ptr] * 6 + \
+        [_c_i64, _ptr, _ptr, _c_i64, _c_i64, _c_i64, _ptr, ctypes.c_int]
+    L.op_sum_payoffs.restype = ctypes.c_int
+    rc = L.op_sum_payoffs(int(contract), 0, n, _p(dv), dv.shape[1], _p(cs), _p(vols),
+                          _p(drifts), _p(sts), _p(detv), _p(disc), K, _p(bbi), _p(bbd),
+                          D, U, M, _p(sums), int(nthreads or os.cpu_count() or 1))
+    if rc != 0:
+        raise ValueError(f"contract {contract} does not take [{D}][{U}] paths")
+    return (sums / n).astype(np.float32)
+
+
+def op_payoffs(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, md_detvals,
+               md_discts, bb_inds, bb_data, lo=0, hi=None, nthreads=None) -> np.ndarray:
+    """The f32 payoffs of paths lo .. hi-1, [M][hi-lo]."""
+    hi = int(num_mc_it if hi is None else hi)
+    dv = np.ascontiguousarray(dir_vs, dtype=np.int32)
+    cs, vols, drifts, sts, detv, disc, bbd = [
+        np.ascontiguousarray(a, dtype=np.float32)
+        for a in (md_cs, md_vols, md_drifts, md_sts, md_detvals, md_discts, bb_data)]
+    bbi = np.ascontiguousarray(bb_inds, dtype=np.int32)
+    M, U = sts.shape
+    D, K = bbi.shape[1], disc.shape[1]
+    out = np.zeros((M, max(hi - lo, 0)), np.float32)
+    L = lib()
+    L.op_payoffs.argtypes = [_c_i32, _c_i64, _c_i64, _ptr, _c_i64] + [_ptr] * 6 + \
+        [_c_i64, _ptr, _ptr, _c_i64, _c_i64, _c_i64, _ptr, ctypes.c_int]
+    L.op_payoffs.restype = ctypes.c_int
+    if L.op_payoffs(int(contract), lo, hi, _p(dv), dv.shape[1], _p(cs), _p(vols), _p(drifts),
+                    _p(sts), _p(detv), _p(disc), K, _p(bbi), _p(bbd), D, U, M, _p(out),
+                    int(nthreads or os.cpu_count() or 1)) != 0:
+        raise ValueError(f"contract {contract} does not take [{D}][{U}] paths")
+    return out

@@ -12,11 +12,19 @@ from conftest import ROOT
 import locvolcalib as lvc
 
 HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
+# Each generated library declares the context and array API beside its own
+# entry points; optionpricing.h belongs to liboptionpricing.so.
+OWN_HEADERS = {"optionpricing.h"}
+SHARED = ("futhark_context.h", "futhark_arrays.h")
 
 
-def header_functions():
+def header_functions(library="locvolcalib"):
     names = set()
-    for h in HEADERS:
+    if library == "locvolcalib":
+        hs = [h for h in HEADERS if os.path.basename(h) not in OWN_HEADERS]
+    else:
+        hs = [os.path.join(ROOT, "include", n) for n in SHARED + (library + ".h",)]
+    for h in hs:
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
         names |= set(re.findall(r"\b((?:futhark|lvc)_[A-Za-z0-9_]+)\s*\(", text))
     return sorted(names)
