@@ -177,6 +177,50 @@ def bench_ocean(lvc, n=57600, m=115, reps=20, seed=3):
     return out
 
 
+def bench_option_pricing(reps=10, cpu=True):
+    """finpar/OptionPricing.fut's `main` on the reference's three datasets
+    (tests/golden/OptionPricing-data: contracts 1, 2, 3), inputs resident on
+    the device: ms per call (entry + sync) and Sobol paths per second; with
+    `cpu`, the oracle's time on the same dataset (OpenMP, 16 threads) and the
+    relative difference of the two results."""
+    import numpy as np
+    from locvolcalib import futhark_values as fv
+    from locvolcalib import optionpricing as op
+    data = os.path.join(ROOT, "tests", "golden", "OptionPricing-data")
+    res = {}
+    with op.OptionPricing(device=0) as ctx:
+        for name in ("small", "medium", "large"):
+            c, n, *arrays = op.main_arguments(fv.load(os.path.join(data, f"{name}.in")))
+            hs = ctx.upload_arguments(arrays)
+            try:
+                for _ in range(3):
+                    ctx.free_f32(ctx.main_raw(c, n, hs))
+                ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    h = ctx.main_raw(c, n, hs)
+                    ctx.sync()
+                    ctx.free_f32(h)
+                dt = (time.perf_counter() - t0) / reps
+                h = ctx.main_raw(c, n, hs)
+                got = ctx.values_f32(h)
+                ctx.free_f32(h)
+            finally:
+                ctx.free_arguments(hs)
+            r = {"contract": c, "paths": n, "ms_per_call": round(dt * 1e3, 4),
+                 "paths_per_s": round(n / dt, 1)}
+            if cpu:
+                from oracle import oracle
+                t0 = time.perf_counter()
+                want = oracle.op_main(c, n, *arrays, nthreads=16)
+                ct = time.perf_counter() - t0
+                r["cpu_oracle"] = {"ms": round(ct * 1e3, 2), "paths_per_s": round(n / ct, 1),
+                                   "cores": 16, "kind": "port"}
+                r["max_rel_err_vs_oracle"] = float(np.max(np.abs(got - want) / np.abs(want)))
+            res[name] = r
+    return res
+
+
 def bench_serving(lvc, rate, duration_s=1.5, seed=7, max_slots=1024, budget=64 << 30):
     """Poisson arrivals at `rate` req/s of single problems (sizes and
     parameters as the batched set), continuous batching, FP32 + refinement.
@@ -324,6 +368,7 @@ def main():
     }
     if not args.no_forms:
         out["ocean_tridiag"] = bench_ocean(lvc)
+        out["option_pricing"] = bench_option_pricing(cpu=not args.no_cpu_baseline)
         out["batched"] = bench_batched(lvc)
         out["serving"] = [bench_serving(lvc, float(r)) for r in args.serve_rates.split(",")]
     if not args.no_cpu_baseline:

@@ -14,13 +14,6 @@
 #include "../../include/locvolcalib.h"
 #include "lvc_runtime.h"
 
-void *lvc_array_mem_f64(const struct futhark_f64_1d *a);
-void *lvc_array_mem_i32(const struct futhark_i32_1d *a);
-int64_t lvc_array_len_f64(const struct futhark_f64_1d *a);
-int64_t lvc_array_len_i32(const struct futhark_i32_1d *a);
-struct futhark_f32_1d *lvc_array_adopt_f32(void *mem, int64_t n);
-struct futhark_f64_1d *lvc_array_adopt_f64(void *mem, int64_t n);
-
 int lvc_validate_sizes(struct futhark_context *ctx, int64_t nx, int64_t ny,
                        int64_t nt) {
   /* LocVolCalib.fut:37 indexes x[i-1], x[i+1] for i in 1...n-2, and the

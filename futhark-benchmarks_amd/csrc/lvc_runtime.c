@@ -49,20 +49,14 @@ int lvc_ready(struct futhark_context *ctx) {
 
 /* ---- configuration ------------------------------------------------------ */
 
-static const char *const tuning_names[] = {
-    "main.suff_intra_par_0", "main.suff_intra_par_1", "lvc.precision",
-    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.graph", "lvc.xshare"};
-static const char *const tuning_classes[] = {
-    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag",
-    "count"};
-#define NTUNING ((int)(sizeof(tuning_names) / sizeof(tuning_names[0])))
-
-int futhark_get_tuning_param_count(void) { return NTUNING; }
+/* The tuning parameters belong to the program (lvc_prog, defined beside
+ * its entry points). */
+int futhark_get_tuning_param_count(void) { return lvc_prog.ntuning; }
 const char *futhark_get_tuning_param_name(int i) {
-  return (i >= 0 && i < NTUNING) ? tuning_names[i] : NULL;
+  return (i >= 0 && i < lvc_prog.ntuning) ? lvc_prog.names[i] : NULL;
 }
 const char *futhark_get_tuning_param_class(int i) {
-  return (i >= 0 && i < NTUNING) ? tuning_classes[i] : NULL;
+  return (i >= 0 && i < lvc_prog.ntuning) ? lvc_prog.classes[i] : NULL;
 }
 
 struct futhark_context_config *futhark_context_config_new(void) {
@@ -107,46 +101,9 @@ void futhark_context_config_set_cache_file(struct futhark_context_config *cfg,
 
 int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
                                             const char *name, size_t v) {
-  if (!cfg || !name)
+  if (!cfg || !name || !lvc_prog.set_param)
     return 1;
-  if (strcmp(name, "main.suff_intra_par_0") == 0) {
-    cfg->suff_intra_par_0 = v;
-    return 0;
-  }
-  if (strcmp(name, "main.suff_intra_par_1") == 0) {
-    cfg->suff_intra_par_1 = v;
-    return 0;
-  }
-  if (strcmp(name, "lvc.precision") == 0) {
-    if (v > LVC_FP64)
-      return 1;
-    cfg->precision = (int)v;
-    return 0;
-  }
-  if (strcmp(name, "lvc.seg_x") == 0 || strcmp(name, "lvc.seg_y") == 0) {
-    if (v != 0 && !lvc_seg_supported((int)v, name[8] == 'y'))
-      return 1;
-    if (name[8] == 'x')
-      cfg->seg_x = (int)v;
-    else
-      cfg->seg_y = (int)v;
-    return 0;
-  }
-  if (strcmp(name, "lvc.xshare") == 0) {
-    if (v > LVC_XSHARE_MAX)
-      return 1;
-    cfg->xshare = (int)v;
-    return 0;
-  }
-  if (strcmp(name, "lvc.graph") == 0) {
-    cfg->graph = v != 0;
-    return 0;
-  }
-  if (strcmp(name, "lvc.autotune") == 0) {
-    cfg->autotune = v != 0;
-    return 0;
-  }
-  return 1;
+  return lvc_prog.set_param(cfg, name, v);
 }
 
 void futhark_context_config_set_debugging(struct futhark_context_config *cfg,
@@ -199,7 +156,8 @@ struct futhark_context *futhark_context_new(struct futhark_context_config *cfg) 
                             ctx->device) != hipSuccess || ctx->num_cus <= 0)
     ctx->num_cus = 256;
   ctx->ok = 1;
-  lvc_tune_load(ctx);
+  if (lvc_prog.context_ready)
+    lvc_prog.context_ready(ctx);
   return ctx;
 }
 

@@ -82,6 +82,18 @@ struct futhark_context {
   int ntuned;
 };
 
+/* What differs between the programs built on this runtime: the tuning
+ * parameters futhark_get_tuning_param_* list, how a configuration takes
+ * them (0: accepted, 1: unknown name or bad value), and a hook run once a
+ * context has opened its device.  Each library defines `lvc_prog`. */
+typedef struct {
+  int ntuning;
+  const char *const *names, *const *classes;
+  int (*set_param)(struct futhark_context_config *cfg, const char *name, size_t v);
+  void (*context_ready)(struct futhark_context *ctx);
+} lvc_program;
+extern const lvc_program lvc_prog;
+
 /* errors: printf-style; keeps the first message until it is read */
 void lvc_set_error(struct futhark_context *ctx, const char *fmt, ...);
 int lvc_check_hip(struct futhark_context *ctx, int err, const char *what);
@@ -95,6 +107,15 @@ void lvc_dev_free(struct futhark_context *ctx, void *ptr, size_t bytes);
  * on ctx->stream and the host data may be reused on return */
 int lvc_upload(struct futhark_context *ctx, void *dst, const void *src,
                size_t bytes);
+
+/* the 1-D arrays' device memory and length; adopt wraps device memory from
+ * lvc_dev_alloc as an array that futhark_free_*_1d returns to the pool */
+void *lvc_array_mem_f64(const struct futhark_f64_1d *a);
+void *lvc_array_mem_i32(const struct futhark_i32_1d *a);
+int64_t lvc_array_len_f64(const struct futhark_f64_1d *a);
+int64_t lvc_array_len_i32(const struct futhark_i32_1d *a);
+struct futhark_f32_1d *lvc_array_adopt_f32(void *mem, int64_t n);
+struct futhark_f64_1d *lvc_array_adopt_f64(void *mem, int64_t n);
 
 /* profiling brackets around a launch (no-ops unless profiling is on) */
 int lvc_prof_begin(struct futhark_context *ctx, const char *name);

@@ -166,3 +166,59 @@ void lvc_tune_pick(struct futhark_context *ctx, int precision, int nx, int ny,
   table_put(ctx, precision, nx, ny, bx, by);
   cache_append(ctx, precision, nx, ny, bx, by);
 }
+
+/* ---- the program's tuning parameters ------------------------------------ */
+
+/* LocVolCalib.fut's two thresholds (the generated API lists them; the
+ * kernels here need no threshold, so they are accepted and kept), then
+ * this library's own knobs. */
+static const char *const tuning_names[] = {
+    "main.suff_intra_par_0", "main.suff_intra_par_1", "lvc.precision",
+    "lvc.seg_x", "lvc.seg_y", "lvc.autotune", "lvc.graph", "lvc.xshare"};
+static const char *const tuning_classes[] = {
+    "threshold", "threshold", "precision", "segment", "segment", "flag", "flag",
+    "count"};
+
+static int set_param(struct futhark_context_config *cfg, const char *name, size_t v) {
+  if (strcmp(name, "main.suff_intra_par_0") == 0) {
+    cfg->suff_intra_par_0 = v;
+    return 0;
+  }
+  if (strcmp(name, "main.suff_intra_par_1") == 0) {
+    cfg->suff_intra_par_1 = v;
+    return 0;
+  }
+  if (strcmp(name, "lvc.precision") == 0) {
+    if (v > LVC_FP64)
+      return 1;
+    cfg->precision = (int)v;
+    return 0;
+  }
+  if (strcmp(name, "lvc.seg_x") == 0 || strcmp(name, "lvc.seg_y") == 0) {
+    if (v != 0 && !lvc_seg_supported((int)v, name[8] == 'y'))
+      return 1;
+    if (name[8] == 'x')
+      cfg->seg_x = (int)v;
+    else
+      cfg->seg_y = (int)v;
+    return 0;
+  }
+  if (strcmp(name, "lvc.xshare") == 0) {
+    if (v > LVC_XSHARE_MAX)
+      return 1;
+    cfg->xshare = (int)v;
+    return 0;
+  }
+  if (strcmp(name, "lvc.graph") == 0) {
+    cfg->graph = v != 0;
+    return 0;
+  }
+  if (strcmp(name, "lvc.autotune") == 0) {
+    cfg->autotune = v != 0;
+    return 0;
+  }
+  return 1;
+}
+
+const lvc_program lvc_prog = {(int)(sizeof(tuning_names) / sizeof(tuning_names[0])),
+                              tuning_names, tuning_classes, set_param, lvc_tune_load};

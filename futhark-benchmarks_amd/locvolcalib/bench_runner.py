@@ -1,9 +1,9 @@
 """A ``futhark bench``-shaped runner for the library's entry points.
 
 Reads the datasets of a program's benchmark stanza -- LocVolCalib.fut:1-10
-(``compiled input @ LocVolCalib-data/small.in`` / ``output @ ...small.out``)
-or misc/ocean-sim/tridiag-test.fut:6-12 (its four entries on
-``data/tridiag32-small.in``) -- in Futhark's text or binary value format,
+(``compiled input @ LocVolCalib-data/small.in`` / ``output @ ...small.out``),
+OptionPricing.fut:1-10, or misc/ocean-sim/tridiag-test.fut:6-12 (its four
+entries on ``data/tridiag32-small.in``) -- in Futhark's text or binary value format,
 runs each entry on each dataset a number of times through the C ABI, checks
 the result against the expected output, and reports per-run wall times in
 microseconds: on stdout as one line per dataset, and with ``--json FILE`` in
@@ -21,6 +21,7 @@ Usage::
 
     python -m locvolcalib.bench_runner --program path/to/finpar/LocVolCalib.fut
     python -m locvolcalib.bench_runner --program path/to/misc/ocean-sim/tridiag-test.fut
+    python -m locvolcalib.bench_runner --program path/to/finpar/OptionPricing.fut
     python -m locvolcalib.bench_runner path/to/small.in path/to/medium.in ...
 
 The expected output of ``X.in`` is ``X.out`` beside it (or the stanza's
@@ -43,6 +44,7 @@ import numpy as np
 from . import futhark_values
 
 PROGRAM_KEY = "finpar/LocVolCalib.fut:main"
+OP_PROGRAM_KEY = "finpar/OptionPricing.fut:main"
 OCEAN_PROGRAM = "misc/ocean-sim/tridiag-test.fut"
 OCEAN_ENTRIES = ("tridagNested", "tridagNestedConst", "tridagNestedSeq", "tridagNestedSeqConst")
 
@@ -216,10 +218,64 @@ def bench_ocean(datasets: List[Tuple[str, Optional[str]]], entries: List[str], r
     return results, all_ok
 
 
+def bench_option_pricing(datasets: List[Tuple[str, Optional[str]]], runs: int = 10,
+                         device: int = 0, names: Optional[List[str]] = None, out=sys.stdout,
+                         rtol: float = 2e-3) -> Tuple[Dict, bool]:
+    """finpar/OptionPricing.fut's ``main`` on its datasets (eleven values:
+    OptionPricing.fut:295-306).  The expected outputs are the f32 sums of a
+    sequential backend; the library sums the f32 payoffs in f64, so results
+    are compared with a relative tolerance (default 2e-3, above the 1.1e-3
+    f32 rounding of the large dataset's expected output)."""
+    from . import optionpricing as op
+
+    results: Dict[str, Dict] = {}
+    all_ok = True
+    with op.OptionPricing(device=device) as ctx:
+        for i, (inp, outp) in enumerate(datasets):
+            name = names[i] if names else inp
+            contract, n, *arrays = op.main_arguments(futhark_values.load(inp))
+            hs = ctx.upload_arguments(arrays)
+            try:
+                ctx.free_f32(ctx.main_raw(contract, n, hs))
+                ctx.sync()
+                times = []
+                got = None
+                for r in range(runs):
+                    t0 = time.perf_counter()
+                    h = ctx.main_raw(contract, n, hs)
+                    ctx.sync()
+                    times.append(int(round((time.perf_counter() - t0) * 1e6)))
+                    if r == runs - 1:
+                        got = ctx.values_f32(h)
+                    ctx.free_f32(h)
+            finally:
+                ctx.free_arguments(hs)
+            status = ""
+            if outp is None:
+                cand = os.path.splitext(inp)[0] + ".out"
+                outp = cand if os.path.exists(cand) else None
+            if outp is not None:
+                expected = np.asarray(futhark_values.load(outp)[0], np.float64)
+                rel = (float(np.max(np.abs(got - expected) / np.abs(expected)))
+                       if expected.size else 0.0)
+                ok = got.shape == expected.shape and rel <= rtol
+                status = f"max rel err {rel:.2e} {'ok' if ok else 'FAILED'}"
+                all_ok &= ok
+            mean = float(np.mean(times))
+            print(f"{name}: {mean:10.0f}us (min: {min(times)}us; {n / (min(times) * 1e-6):.3g} "
+                  f"paths/s) {status}", file=out, flush=True)
+            results[name] = {"runtimes": times,
+                             "stderr": "" if not status.endswith("FAILED") else status}
+    return {OP_PROGRAM_KEY: {"datasets": results}}, all_ok
+
+
 def main(argv: Optional[List[str]] = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("datasets", nargs="*", help="dataset .in files")
-    ap.add_argument("--program", help="the reference's LocVolCalib.fut: run its stanza's datasets")
+    ap.add_argument("--program", help="the reference's LocVolCalib.fut, OptionPricing.fut or "
+                    "tridiag-test.fut: run its stanza's datasets")
+    ap.add_argument("--option-pricing", action="store_true",
+                    help="the datasets given are OptionPricing.fut's")
     ap.add_argument("--runs", type=int, default=10)
     ap.add_argument("--precision", choices=("fp32", "fp32r", "fp64"), default="fp32r")
     ap.add_argument("--tolerance", type=float, default=5e-6)
@@ -229,7 +285,9 @@ def main(argv: Optional[List[str]] = None) -> int:
     pairs: List[Tuple[str, Optional[str]]] = []
     names: List[str] = []
     entries = ["main"]
+    option_pricing = a.option_pricing
     if a.program:
+        option_pricing |= os.path.basename(a.program) == "OptionPricing.fut"
         base = os.path.dirname(os.path.abspath(a.program))
         with open(a.program) as f:
             text = f.read()
@@ -243,7 +301,9 @@ def main(argv: Optional[List[str]] = None) -> int:
         names.append(d)
     if not pairs:
         ap.error("no datasets: give .in files or --program")
-    if entries != ["main"]:
+    if option_pricing:
+        res, ok = bench_option_pricing(pairs, runs=a.runs, device=a.device, names=names)
+    elif entries != ["main"]:
         res, ok = bench_ocean(pairs, entries, runs=a.runs, device=a.device, names=names)
     else:
         prec = {"fp32": 0, "fp32r": 1, "fp64": 2}[a.precision]

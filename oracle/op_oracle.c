@@ -110,6 +110,18 @@ static void bb_dates(int64_t D, const int32_t *bb_inds, const float *bb_data,
   for (int64_t i = 0; i < D; i++) out[i * stride] = row[i];
 }
 
+/* brownianBridgeDates (:145-183) of one row of D Gaussians, for the tests.
+ * Returns 1 if the scratch row cannot be allocated. */
+int op_bridge(int64_t D, const int32_t *bb_inds, const float *bb_data, const float *gauss,
+              float *out) {
+  float *row = malloc((size_t)(D > 0 ? D : 1) * sizeof(float));
+  if (!row)
+    return 1;
+  bb_dates(D, bb_inds, bb_data, gauss, 1, row, out);
+  free(row);
+  return 0;
+}
+
 /* OptionPricing.fut:190-221 (correlateDeltas, combineVs, mkPrices) for one
  * model: bb [D][U] -> prices [D][U]. */
 static void black_scholes(int64_t D, int64_t U, const float *bb, const float *md_c,

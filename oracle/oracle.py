@@ -268,6 +268,23 @@ def op_ugaussian(p: float) -> float:
     return float(L.op_ugaussian(ctypes.c_float(p)))
 
 
+def op_bridge(bb_inds: np.ndarray, bb_data: np.ndarray, gauss: np.ndarray) -> np.ndarray:
+    """brownianBridgeDates (:145-183) of one row of num_dates Gaussians."""
+    bbi = np.ascontiguousarray(bb_inds, dtype=np.int32)
+    bbd = np.ascontiguousarray(bb_data, dtype=np.float32)
+    g = np.ascontiguousarray(gauss, dtype=np.float32)
+    D = bbi.shape[1]
+    if bbd.shape != (3, D) or g.shape != (D,):
+        raise ValueError("bb_inds, bb_data [3][D] and gauss [D]")
+    out = np.empty(D, np.float32)
+    L = lib()
+    L.op_bridge.argtypes = [_c_i64, _ptr, _ptr, _ptr, _ptr]
+    L.op_bridge.restype = ctypes.c_int
+    if L.op_bridge(D, _p(bbi), _p(bbd), _p(g), _p(out)) != 0:
+        raise MemoryError("op_bridge")
+    return out
+
+
 def op_main(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, md_detvals,
             md_discts, bb_inds, bb_data, paths=None, nthreads=None) -> np.ndarray:
     """main (:295-322) over the first `paths` paths (default num_mc_it): the

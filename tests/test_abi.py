@@ -10,6 +10,7 @@ import pytest
 
 from conftest import ROOT
 import locvolcalib as lvc
+from locvolcalib import optionpricing as op
 
 HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 # Each generated library declares the context and array API beside its own
@@ -135,3 +136,53 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(lvc, "library_path", str(tmp_path / "missing.so"))
     with pytest.raises(lvc.LocVolCalibError, match="missing"):
         lvc.lib()
+
+
+# ---- liboptionpricing.so -------------------------------------------------
+
+def test_optionpricing_exports_every_declared_function():
+    L = op.lib()
+    names = header_functions("optionpricing")
+    assert "futhark_entry_main" in names and "futhark_new_f32_3d" in names
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in the header but not exported"
+
+
+def test_optionpricing_binding_covers_header():
+    assert set(header_functions("optionpricing")) == set(op.exported_functions())
+
+
+def test_optionpricing_only_public_symbols_exported():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", op.library_path],
+                         capture_output=True, text=True, check=True).stdout
+    syms = [l.split()[-1] for l in out.splitlines() if l.strip()]
+    assert syms and all(s.startswith("futhark_") for s in syms), syms
+
+
+def test_optionpricing_has_no_tuning_params():
+    L = op.lib()
+    assert L.futhark_get_tuning_param_count() == 0
+    assert L.futhark_get_tuning_param_name(0) is None
+    cfg = L.futhark_context_config_new()
+    assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.precision", 1) != 0
+    L.futhark_context_config_free(cfg)
+
+
+def test_optionpricing_without_device_reports_error():
+    L = op.lib()
+    cfg = L.futhark_context_config_new()
+    ctx = L.futhark_context_new(cfg)
+    try:
+        err = lvc._take_string(L.futhark_context_get_error(ctx))
+        if err is None:
+            pytest.skip("a GPU is present; this checks the no-device path")
+        out = ctypes.c_void_p()
+        rc = L.futhark_entry_main(ctx, ctypes.byref(out), 1, 8, *([None] * 9))
+        assert rc != 0 and not out.value
+        assert "device" in lvc._take_string(L.futhark_context_get_error(ctx)).lower()
+        with pytest.raises(op.OptionPricingError):
+            op.OptionPricing()
+    finally:
+        L.futhark_context_free(ctx)
+        L.futhark_context_config_free(cfg)

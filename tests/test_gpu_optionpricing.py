@@ -1,0 +1,183 @@
+"""OptionPricing on the GPU (liboptionpricing.so through its C ABI) against
+the oracle's f64-accumulated mean of the reference's f32 payoffs, on the
+reference's three datasets and on cases cut from them.
+
+Tolerance: the kernels compute each path's payoff in f32 in the
+reference's order of operations; expf, logf and sqrtf on the device may
+differ from the host's libm by an ulp, which moves a payoff by a few ulps
+(and, at the rare path that sits on a payoff barrier, flips it), so the
+means agree to rtol 2e-5 rather than bit for bit.  Against the expected
+outputs of the datasets, which carry the f32 rounding of a sequential sum
+over the paths, the tolerance is rtol 2e-3.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import oracle
+from locvolcalib import futhark_values as fv
+from locvolcalib import optionpricing as op
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(GOLDEN, "OptionPricing-data")
+RTOL = 2e-5
+
+
+def load(name):
+    args = op.main_arguments(fv.load(os.path.join(DATA, f"{name}.in")))
+    expected = np.asarray(fv.load(os.path.join(DATA, f"{name}.out"))[0], np.float32)
+    return args, expected
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = op.OptionPricing(device=0)
+    yield c
+    c.free()
+
+
+@pytest.fixture(scope="module")
+def datasets():
+    return {n: load(n) for n in ("small", "medium", "large")}
+
+
+@pytest.mark.parametrize("name", ["small", "medium", "large"])
+def test_dataset(ctx, datasets, name):
+    args, expected = datasets[name]
+    print(f"OptionPricing {name}: contract {args[0]}, {args[1]} paths", flush=True)
+    got = ctx.main(*args)
+    assert got.dtype == np.float32 and got.shape == expected.shape
+    np.testing.assert_allclose(got, oracle.op_main(*args), rtol=RTOL)
+    np.testing.assert_allclose(got, expected, rtol=2e-3)
+
+
+@pytest.mark.parametrize("name", ["small", "medium", "large"])
+@pytest.mark.parametrize("paths", [1, 2, 63, 64, 65, 255, 256, 257, 1000, 4099])
+def test_path_counts(ctx, datasets, name, paths):
+    """Paths 1 .. n, n around the wave (64) and workgroup (256) blocks the
+    kernel splits the Sobol indices into."""
+    args, _ = datasets[name]
+    got = ctx.main(args[0], paths, *args[2:])
+    np.testing.assert_allclose(got, oracle.op_main(args[0], paths, *args[2:]), rtol=RTOL,
+                               atol=1e-4)
+
+
+def test_grid_stride(ctx, datasets):
+    """More blocks of 256 paths than the grid has workgroups (2048): the
+    grid-stride loop, on contract 2 (register-resident path)."""
+    args, _ = datasets["medium"]
+    n = 2048 * 256 * 2 + 77
+    got = ctx.main(args[0], n, *args[2:])
+    np.testing.assert_allclose(got, oracle.op_main(args[0], n, *args[2:]), rtol=RTOL)
+
+
+def _models(args, m, seed):
+    """m market models from the dataset's one: vols, drifts, starts and
+    discounts perturbed per model."""
+    rng = np.random.default_rng(seed)
+    c, n, dv, cs, vols, drifts, sts, detv, disc, bbi, bbd = args
+    scale = lambda a, s: (a * (1 + s * rng.standard_normal((m,) + a.shape[1:]))).astype(np.float32)
+    return (c, n, dv, np.repeat(cs, m, 0), scale(np.repeat(vols, m, 0), 0.1),
+            scale(np.repeat(drifts, m, 0), 0.1), scale(np.repeat(sts, m, 0), 0.02),
+            np.repeat(detv, m, 0), scale(np.repeat(disc, m, 0), 0.01), bbi, bbd)
+
+
+@pytest.mark.parametrize("name,m", [("small", 8), ("medium", 3), ("large", 2)])
+def test_several_models(ctx, datasets, name, m):
+    args = _models(datasets[name][0], m, seed=m)
+    n = 20000
+    got = ctx.main(args[0], n, *args[2:])
+    assert got.shape == (m,)
+    np.testing.assert_allclose(got, oracle.op_main(args[0], n, *args[2:]), rtol=RTOL)
+
+
+@pytest.mark.parametrize("bits", [1, 5, 6, 7, 16])
+def test_fewer_direction_bits(ctx, datasets, bits):
+    """num_bits below and around the 6 low Gray-code bits a lane owns."""
+    args, _ = datasets["medium"]
+    dv = np.ascontiguousarray(args[2][:, :bits])
+    n = 3000
+    got = ctx.main(args[0], n, dv, *args[3:])
+    np.testing.assert_allclose(got, oracle.op_main(args[0], n, dv, *args[3:]), rtol=RTOL)
+
+
+def test_31_direction_bits(ctx, datasets):
+    args, _ = datasets["medium"]
+    rng = np.random.default_rng(31)
+    dv = rng.integers(0, 1 << 31, size=(15, 31), dtype=np.int64).astype(np.int32)
+    dv[:, 30] = 1 << 30
+    n = 5000
+    got = ctx.main(args[0], n, dv, *args[3:])
+    np.testing.assert_allclose(got, oracle.op_main(args[0], n, dv, *args[3:]), rtol=RTOL)
+
+
+def test_repeated_calls_identical(ctx, datasets):
+    """The sum order is fixed by the path count: a result does not change
+    from call to call."""
+    args, _ = datasets["large"]
+    a = ctx.main(args[0], 50000, *args[2:])
+    b = ctx.main(args[0], 50000, *args[2:])
+    np.testing.assert_array_equal(a, b)
+
+
+def test_no_paths_is_nan(ctx, datasets):
+    """num_mc_it = 0: the empty sum divided by 0 (:322)."""
+    args, _ = datasets["medium"]
+    assert np.all(np.isnan(ctx.main(args[0], 0, *args[2:])))
+
+
+def test_other_contract_is_zero(ctx, datasets):
+    """A contract number without a payoff prices at 0 (:294), whatever the
+    shapes."""
+    args, _ = datasets["medium"]
+    np.testing.assert_array_equal(ctx.main(7, 100, *args[2:]), np.zeros(1, np.float32))
+
+
+def test_no_models(ctx, datasets):
+    args, _ = datasets["medium"]
+    c, n, dv, cs, vols, drifts, sts, detv, disc, bbi, bbd = args
+    got = ctx.main(c, 100, dv, cs[:0], vols[:0], drifts[:0], sts[:0], detv[:0], disc[:0],
+                   bbi, bbd)
+    assert got.shape == (0,)
+
+
+@pytest.mark.parametrize("case,fragment", [
+    ("contract", "contract 1 takes"),
+    ("negative", "negative"),
+    ("bits", "at most 31"),
+    ("models", "at most 8"),
+    ("discounts", "discounts"),
+    ("shape", "md_vols"),
+])
+def test_errors(ctx, datasets, case, fragment):
+    args, _ = datasets["medium"]
+    c, n, dv, cs, vols, drifts, sts, detv, disc, bbi, bbd = args
+    if case == "contract":
+        c = 1
+    elif case == "negative":
+        n = -1
+    elif case == "bits":
+        dv = np.zeros((15, 32), np.int32)
+    elif case == "models":
+        c, n, dv, cs, vols, drifts, sts, detv, disc, bbi, bbd = _models(args, 9, 0)
+    elif case == "discounts":
+        disc = disc[:, :4]
+    elif case == "shape":
+        vols = vols[:, :4]
+    with pytest.raises(op.OptionPricingError, match=fragment):
+        ctx.main(c, n, dv, cs, vols, drifts, sts, detv, disc, bbi, bbd)
+    # the context stays usable after an error
+    got = ctx.main(args[0], 64, *args[2:])
+    np.testing.assert_allclose(got, oracle.op_main(args[0], 64, *args[2:]), rtol=RTOL)
+
+
+def test_both_libraries_in_one_process(ctx, datasets, gpu_ctx):
+    """liblocvolcalib.so and liboptionpricing.so export the same futhark_*
+    names; each context keeps working beside the other."""
+    args, _ = datasets["small"]
+    got = ctx.main(args[0], 1000, *args[2:])
+    np.testing.assert_allclose(got, oracle.op_main(args[0], 1000, *args[2:]), rtol=RTOL)
+    assert np.isfinite(gpu_ctx.main(2, 32, 32, 8, 0.03, 5.0, 0.2, 0.6, 0.5)).all()
