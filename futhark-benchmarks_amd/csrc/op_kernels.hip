@@ -221,18 +221,32 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *p
     const int32_t *bi = A.bbi, *li = A.bbi + D, *ri = A.bbi + 2 * D;
     const float *sd = A.bbd, *lw = A.bbd + D, *rw = A.bbd + 2 * D;
     auto clamp = [](int i) { return i < 0 ? 0 : (i >= D ? D - 1 : i); };
+    // The U rows are independent chains: each step reads all of them,
+    // then computes, then writes, so a lane overlaps U chains' memory
+    // latency (a step's own reads precede its write, as in :157-167).
+    for (int i = 0; i < D; ++i) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      float *row = rows[u];
-      for (int i = 0; i < D; ++i) row[i] = 0.0f;
-      row[clamp(bi[0] - 1)] = sd[0] * gauss(u);
-      for (int i = 1; i < D; ++i) {
-        const int j = li[i] - 1, k = clamp(ri[i] - 1), l = clamp(bi[i] - 1);
-        const float wk = row[k];
-        const float zi = gauss(i * U + u);
-        const float tmp = rw[i] * wk + sd[i] * zi;
-        const float rj = row[clamp(j)];
-        row[l] = j == -1 ? tmp : tmp + lw[i] * rj;
+      for (int u = 0; u < U; ++u) rows[u][i] = 0.0f;
+    }
+    {
+      const int l0 = clamp(bi[0] - 1);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rows[u][l0] = sd[0] * gauss(u);
+    }
+    for (int i = 1; i < D; ++i) {
+      const int j = li[i] - 1, k = clamp(ri[i] - 1), l = clamp(bi[i] - 1), jc = clamp(j);
+      float wk[U], rj[U], zi[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        wk[u] = rows[u][k];
+        rj[u] = rows[u][jc];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) zi[u] = gauss(i * U + u);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float tmp = rw[i] * wk[u] + sd[i] * zi[u];
+        rows[u][l] = j == -1 ? tmp : tmp + lw[i] * rj[u];
       }
     }
     for (int m = 0; m < A.M; ++m) {
