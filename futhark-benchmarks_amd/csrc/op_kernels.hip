@@ -98,15 +98,24 @@ __device__ __forceinline__ float fmin_payoff(float x0, float x1, float x2) {
   return a < b ? (a < c ? a : c) : (b < c ? b : c);
 }
 
+// One market model set (OptionPricing.fut:299-304), row-major; the kernel's
+// __restrict__ parameters, so that wave-uniform reads of them become scalar
+// loads.
+struct Market {
+  const float *cs, *vols, *drifts, *sts, *detvals, *discts;
+  int K;
+};
+
 // The payoff of one path under model m (Black-Scholes :190-225 streamed
 // date by date into payoff1-3, :238-293).  rows: the bridge rows [U][D]
 // before the differences of :178-182, which are taken here.
 template <int C>
-__device__ __forceinline__ float model_payoff(const op_args &A, int m, const float (*rows)[Shape<C>::D]) {
+__device__ __forceinline__ float model_payoff(const Market &mk, int m,
+                                              const float (*rows)[Shape<C>::D]) {
   constexpr int D = Shape<C>::D, U = Shape<C>::U;
-  const float *cs = A.cs + m * U * U, *vols = A.vols + m * D * U;
-  const float *drifts = A.drifts + m * D * U, *sts = A.sts + m * U;
-  const float *disc = A.discts + m * A.K;
+  const float *cs = mk.cs + m * U * U, *vols = mk.vols + m * D * U;
+  const float *drifts = mk.drifts + m * D * U, *sts = mk.sts + m * U;
+  const float *disc = mk.discts + m * mk.K;
   float acc[U], price[U], prev[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -142,7 +151,7 @@ __device__ __forceinline__ float model_payoff(const op_args &A, int m, const flo
              price[2] <= 840.0f;
   }
   if constexpr (C == 1) {  // payoff1 (:240-244)
-    const float amount = (price[0] - 4000.0f) * A.detvals[m];
+    const float amount = (price[0] - 4000.0f) * mk.detvals[m];
     const float amount0 = 0.0f < amount ? amount : 0.0f;
     return amount0 * disc[0];
   } else if constexpr (C == 2) {  // payoff2 (:246-265)
@@ -177,7 +186,14 @@ __device__ __forceinline__ float model_payoff(const op_args &A, int m, const flo
 }
 
 template <int C>
-__global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *partial) {
+__global__ __launch_bounds__(kThreads) void op_paths_kernel(
+    const int32_t *__restrict__ dv, const int32_t *__restrict__ bbi,
+    const float *__restrict__ bbd, const float *__restrict__ cs,
+    const float *__restrict__ vols, const float *__restrict__ drifts,
+    const float *__restrict__ sts, const float *__restrict__ detvals,
+    const float *__restrict__ discts, int bits, int K, int M, int64_t N,
+    double *__restrict__ partial) {
+  const Market mk = {cs, vols, drifts, sts, detvals, discts, K};
   constexpr int D = Shape<C>::D, U = Shape<C>::U, DIMS = D * U;
   __shared__ int32_t xhi[kWaves][DIMS];
   __shared__ double red[kThreads];
@@ -186,10 +202,10 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *p
   double sum[OP_MAX_MODELS];
 #pragma unroll
   for (int m = 0; m < OP_MAX_MODELS; ++m) sum[m] = 0.0;
-  const float divisor = (float)(1u << A.bits);  // 2 ** num_bits (:31), bits <= 31
+  const float divisor = (float)(1u << bits);  // 2 ** num_bits (:31), bits <= 31
   float rows[U][D];
   // blocks of kThreads Sobol indices covering 0 .. N (index 0 is no path)
-  const int64_t nblk = A.N / kThreads + 1;
+  const int64_t nblk = N / kThreads + 1;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const int64_t n0 = blk * kThreads;
     __syncthreads();  // the previous block's readers of xhi are done
@@ -197,7 +213,7 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *p
       const int w = e / DIMS, d = e - w * DIMS;
       const uint32_t c = (uint32_t)(n0 + 64 * w);
       const uint32_t g = c ^ (c >> 1);  // grayCode (:12) of the wave's base
-      const int32_t *v = A.dv + d * OP_DV_STRIDE;
+      const int32_t *v = dv + d * OP_DV_STRIDE;
       int32_t x = 0;
 #pragma unroll
       for (int b = 0; b < OP_DV_STRIDE; ++b) x ^= ((g >> b) & 1u) ? v[b] : 0;
@@ -205,10 +221,10 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *p
     }
     __syncthreads();
     const int64_t n = n0 + tid;  // Sobol index of path n - 1 (:314)
-    const bool live = n >= 1 && n <= A.N;
+    const bool live = n >= 1 && n <= N;
     // the Gaussian of Sobol dimension d of this path (:29-35, :121-140)
     auto gauss = [&](int d) {
-      const int32_t *v = A.dv + d * OP_DV_STRIDE;
+      const int32_t *v = dv + d * OP_DV_STRIDE;
       int32_t x = xhi[wave][d];
 #pragma unroll
       for (int b = 0; b < 6; ++b) x ^= ((glo >> b) & 1u) ? v[b] : 0;
@@ -218,8 +234,8 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *p
     // of dimensions i * U + u (the [num_dates][num_und] reshape, :311)
     // (the reference reads bb_inds unchecked, :156; indexes are clamped to
     // the row so that no input makes an access out of bounds)
-    const int32_t *bi = A.bbi, *li = A.bbi + D, *ri = A.bbi + 2 * D;
-    const float *sd = A.bbd, *lw = A.bbd + D, *rw = A.bbd + 2 * D;
+    const int32_t *bi = bbi, *li = bbi + D, *ri = bbi + 2 * D;
+    const float *sd = bbd, *lw = bbd + D, *rw = bbd + 2 * D;
     auto clamp = [](int i) { return i < 0 ? 0 : (i >= D ? D - 1 : i); };
     // The U rows are independent chains: each step reads all of them,
     // then computes, then writes, so a lane overlaps U chains' memory
@@ -249,20 +265,20 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(op_args A, double *p
         rows[u][l] = j == -1 ? tmp : tmp + lw[i] * rj[u];
       }
     }
-    for (int m = 0; m < A.M; ++m) {
-      const float pay = model_payoff<C>(A, m, rows);
+    for (int m = 0; m < M; ++m) {
+      const float pay = model_payoff<C>(mk, m, rows);
       if (live) sum[m] += (double)pay;
     }
   }
   // the workgroup's sums, model by model, by a fixed tree
-  for (int m = 0; m < A.M; ++m) {
+  for (int m = 0; m < M; ++m) {
     red[tid] = sum[m];
     __syncthreads();
     for (int s = kThreads / 2; s > 0; s >>= 1) {
       if (tid < s) red[tid] += red[tid + s];
       __syncthreads();
     }
-    if (tid == 0) partial[(int64_t)blockIdx.x * A.M + m] = red[0];
+    if (tid == 0) partial[(int64_t)blockIdx.x * M + m] = red[0];
     __syncthreads();
   }
 }
@@ -312,14 +328,16 @@ int op_launch(void *stream, int contract, const op_args *A, const int32_t *dir_v
     return (int)hipErrorInvalidValue;
   const int np = dims * OP_DV_STRIDE;
   op_pad_kernel<<<(np + 255) / 256, 256, 0, st>>>(dir_vs, dv, dims, A->bits);
-  op_args a = *A;
-  a.dv = dv;
   const int nblocks = op_blocks(A->N);
+#define OP_PATHS_ARGS                                                              \
+  dv, A->bbi, A->bbd, A->cs, A->vols, A->drifts, A->sts, A->detvals, A->discts, \
+      A->bits, A->K, A->M, A->N, partial
   switch (contract) {
-    case 1: op_paths_kernel<1><<<nblocks, kThreads, 0, st>>>(a, partial); break;
-    case 2: op_paths_kernel<2><<<nblocks, kThreads, 0, st>>>(a, partial); break;
-    default: op_paths_kernel<3><<<nblocks, kThreads, 0, st>>>(a, partial); break;
+    case 1: op_paths_kernel<1><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
+    case 2: op_paths_kernel<2><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
+    default: op_paths_kernel<3><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
   }
+#undef OP_PATHS_ARGS
   op_mean_kernel<<<1, kThreads, 0, st>>>(partial, nblocks, A->M, A->N, out);
   return (int)hipGetLastError();
 }
