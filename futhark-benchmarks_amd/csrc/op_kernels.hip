@@ -57,39 +57,101 @@ __device__ __forceinline__ float poly_appr(float x, const float (&a)[8], const f
 }
 
 // OptionPricing.fut:121-136 (ugaussianEl) with the coefficients of
-// smallcase (:58-79), intermediate (:81-99) and tail (:101-119).
-__device__ __forceinline__ float ugaussian(float p) {
-  constexpr float sa[8] = {3.387132872796366608f, 133.14166789178437745f,
-                           1971.5909503065514427f, 13731.693765509461125f,
-                           45921.953931549871457f, 67265.770927008700853f,
-                           33430.575583588128105f, 2509.0809287301226727f};
-  constexpr float sb[8] = {1.0f, 42.313330701600911252f, 687.1870074920579083f,
-                           5394.1960214247511077f, 21213.794301586595867f,
-                           39307.89580009271061f, 28729.085735721942674f,
-                           5226.495278852854561f};
-  constexpr float ia[8] = {1.42343711074968357734f, 4.6303378461565452959f,
-                           5.7694972214606914055f, 3.64784832476320460504f,
-                           1.27045825245236838258f, 0.24178072517745061177f,
-                           0.0227238449892691845833f, 7.7454501427834140764e-4f};
-  constexpr float ib[8] = {1.0f, 2.05319162663775882187f, 1.6763848301838038494f,
-                           0.68976733498510000455f, 0.14810397642748007459f,
-                           0.0151986665636164571966f, 5.475938084995344946e-4f,
-                           1.05075007164441684324e-9f};
-  constexpr float ta[8] = {6.6579046435011037772f, 5.4637849111641143699f,
-                           1.7848265399172913358f, 0.29656057182850489123f,
-                           0.026532189526576123093f, 0.0012426609473880784386f,
-                           2.71155556874348757815e-5f, 2.01033439929228813265e-7f};
-  constexpr float tb[8] = {1.0f, 0.59983220655588793769f, 0.13692988092273580531f,
-                           0.0148753612908506148525f, 7.868691311456132591e-4f,
-                           1.8463183175100546818e-5f, 1.4215117583164458887e-7f,
-                           2.04426310338993978564e-5f};
-  const float dp = p - 0.5f;
-  if ((dp < 0.0f && 0.0f - dp <= 0.425f) || (0.0f <= dp && dp <= 0.425f))
-    return dp * poly_appr(0.180625f - dp * dp, sa, sb);
-  const float pp = dp < 0.0f ? dp + 0.5f : 0.5f - dp;
+// smallcase (:58-79), intermediate (:81-99) and tail (:101-119), in three
+// parts: the branch test on dp = p - 0.5, the central value, and the tail
+// value before its sign (a function of pp alone).
+constexpr float kSa[8] = {3.387132872796366608f, 133.14166789178437745f,
+                          1971.5909503065514427f, 13731.693765509461125f,
+                          45921.953931549871457f, 67265.770927008700853f,
+                          33430.575583588128105f, 2509.0809287301226727f};
+constexpr float kSb[8] = {1.0f, 42.313330701600911252f, 687.1870074920579083f,
+                          5394.1960214247511077f, 21213.794301586595867f,
+                          39307.89580009271061f, 28729.085735721942674f,
+                          5226.495278852854561f};
+constexpr float kIa[8] = {1.42343711074968357734f, 4.6303378461565452959f,
+                          5.7694972214606914055f, 3.64784832476320460504f,
+                          1.27045825245236838258f, 0.24178072517745061177f,
+                          0.0227238449892691845833f, 7.7454501427834140764e-4f};
+constexpr float kIb[8] = {1.0f, 2.05319162663775882187f, 1.6763848301838038494f,
+                          0.68976733498510000455f, 0.14810397642748007459f,
+                          0.0151986665636164571966f, 5.475938084995344946e-4f,
+                          1.05075007164441684324e-9f};
+constexpr float kTa[8] = {6.6579046435011037772f, 5.4637849111641143699f,
+                          1.7848265399172913358f, 0.29656057182850489123f,
+                          0.026532189526576123093f, 0.0012426609473880784386f,
+                          2.71155556874348757815e-5f, 2.01033439929228813265e-7f};
+constexpr float kTb[8] = {1.0f, 0.59983220655588793769f, 0.13692988092273580531f,
+                          0.0148753612908506148525f, 7.868691311456132591e-4f,
+                          1.8463183175100546818e-5f, 1.4215117583164458887e-7f,
+                          2.04426310338993978564e-5f};
+
+__device__ __forceinline__ bool ugaussian_central(float dp) {
+  return (dp < 0.0f && 0.0f - dp <= 0.425f) || (0.0f <= dp && dp <= 0.425f);
+}
+__device__ __forceinline__ float ugaussian_small(float dp) {
+  return dp * poly_appr(0.180625f - dp * dp, kSa, kSb);
+}
+__device__ __forceinline__ float ugaussian_pp(float dp) {
+  return dp < 0.0f ? dp + 0.5f : 0.5f - dp;
+}
+__device__ __forceinline__ float ugaussian_tail(float pp) {
   const float r = sqrtf(-logf(pp));
-  const float x = r <= 5.0f ? poly_appr(r - 1.6f, ia, ib) : poly_appr(r - 5.0f, ta, tb);
+  return r <= 5.0f ? poly_appr(r - 1.6f, kIa, kIb) : poly_appr(r - 5.0f, kTa, kTb);
+}
+
+__device__ __forceinline__ float ugaussian(float p) {
+  const float dp = p - 0.5f;
+  if (ugaussian_central(dp)) return ugaussian_small(dp);
+  const float x = ugaussian_tail(ugaussian_pp(dp));
   return dp < 0.0f ? 0.0f - x : x;
+}
+
+// ugaussian of U numbers per lane, with the tail cases of the whole wave
+// packed together: about 15 % of the numbers take the tail branch, so a
+// wave would run the tail's log, sqrt, polynomial and division once per
+// number with most lanes idle.  Here the tail inputs of all U numbers are
+// written to LDS (`buf`, U * 64 floats of this wave) at slots given by a
+// ballot, evaluated 64 at a time, and read back; every value goes through
+// the same operations as in ugaussian().  All 64 lanes must be active.
+template <int U>
+__device__ __forceinline__ void ugaussian_wave(const float (&p)[U], float (&out)[U],
+                                               float *buf, int lane) {
+  float dp[U];
+  bool tail[U];
+  int slot[U];
+  int total = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    dp[u] = p[u] - 0.5f;
+    tail[u] = !ugaussian_central(dp[u]);
+    out[u] = ugaussian_small(dp[u]);  // replaced below for the tail lanes
+    const uint64_t mask = __ballot(tail[u]);
+    slot[u] = total + (int)__builtin_amdgcn_mbcnt_hi(
+                          (uint32_t)(mask >> 32),
+                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    total += __popcll(mask);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (tail[u]) buf[slot[u]] = ugaussian_pp(dp[u]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int t = lane; t < total; t += 64) buf[t] = ugaussian_tail(buf[t]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (tail[u]) {
+      const float x = buf[slot[u]];
+      out[u] = dp[u] < 0.0f ? 0.0f - x : x;
+    }
+  }
+  // the next call's writes wait for these reads
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // OptionPricing.fut:230-236 (fminPayoff)
@@ -196,6 +258,7 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
   const Market mk = {cs, vols, drifts, sts, detvals, discts, K};
   constexpr int D = Shape<C>::D, U = Shape<C>::U, DIMS = D * U;
   __shared__ int32_t xhi[kWaves][DIMS];
+  __shared__ float tailbuf[kWaves][U * 64];
   __shared__ double red[kThreads];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t glo = (uint32_t)(lane ^ (lane >> 1));
@@ -223,12 +286,25 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
     const int64_t n = n0 + tid;  // Sobol index of path n - 1 (:314)
     const bool live = n >= 1 && n <= N;
     // the Gaussian of Sobol dimension d of this path (:29-35, :121-140)
-    auto gauss = [&](int d) {
+    // the Sobol number of dimension d of this path (:29-35)
+    auto sobol = [&](int d) {
       const int32_t *v = dv + d * OP_DV_STRIDE;
       int32_t x = xhi[wave][d];
 #pragma unroll
       for (int b = 0; b < 6; ++b) x ^= ((glo >> b) & 1u) ? v[b] : 0;
-      return ugaussian((float)x / divisor);
+      return (float)x / divisor;
+    };
+    // the Gaussians (:121-140) of dimensions i * U .. i * U + U - 1
+    auto gauss_date = [&](int i, float(&z)[U]) {
+      float p[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) p[u] = sobol(i * U + u);
+      if constexpr (U > 1) {
+        ugaussian_wave<U>(p, z, tailbuf[wave], lane);
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) z[u] = ugaussian(p[u]);
+      }
     };
     // the Brownian bridge of each underlying (:145-177) over the Gaussians
     // of dimensions i * U + u (the [num_dates][num_und] reshape, :311)
@@ -246,8 +322,10 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
     }
     {
       const int l0 = clamp(bi[0] - 1);
+      float z0[U];
+      gauss_date(0, z0);
 #pragma unroll
-      for (int u = 0; u < U; ++u) rows[u][l0] = sd[0] * gauss(u);
+      for (int u = 0; u < U; ++u) rows[u][l0] = sd[0] * z0[u];
     }
     for (int i = 1; i < D; ++i) {
       const int j = li[i] - 1, k = clamp(ri[i] - 1), l = clamp(bi[i] - 1), jc = clamp(j);
@@ -257,8 +335,7 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
         wk[u] = rows[u][k];
         rj[u] = rows[u][jc];
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u) zi[u] = gauss(i * U + u);
+      gauss_date(i, zi);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const float tmp = rw[i] * wk[u] + sd[i] * zi[u];

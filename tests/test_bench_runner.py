@@ -103,3 +103,40 @@ def test_runner_on_ocean_dataset(tmp_path):
     out = tmp_path / "o.json"
     assert bench_runner.main(["--program", str(prog), "--runs", "1", "--json", str(out)]) == 0
     assert len(json.loads(out.read_text())) == 4
+
+
+# A stanza in OptionPricing.fut's layout (:1-10): three datasets, each with
+# its expected output.
+OP_STANZA = """-- Generic pricing
+-- ==
+-- compiled input @ OptionPricing-data/small.in
+-- output @ OptionPricing-data/small.out
+--
+-- compiled input @ OptionPricing-data/medium.in
+-- output @ OptionPricing-data/medium.out
+--
+-- compiled input @ OptionPricing-data/large.in
+-- output @ OptionPricing-data/large.out
+"""
+
+
+def test_parse_option_pricing_stanza():
+    pairs = bench_runner.parse_stanza(OP_STANZA)
+    assert [p[0] for p in pairs] == [f"OptionPricing-data/{n}.in"
+                                     for n in ("small", "medium", "large")]
+    assert all(o == i[:-3] + ".out" for i, o in pairs)
+    assert bench_runner.parse_entries(OP_STANZA) == ["main"]
+
+
+@pytest.mark.gpu
+def test_runner_on_option_pricing_datasets(tmp_path):
+    print("case: bench runner, OptionPricing small/medium/large", flush=True)
+    prog = tmp_path / "OptionPricing.fut"
+    prog.write_text(OP_STANZA)
+    os.symlink(os.path.join(GOLDEN, "OptionPricing-data"), tmp_path / "OptionPricing-data")
+    out = tmp_path / "o.json"
+    assert bench_runner.main(["--program", str(prog), "--runs", "2", "--json", str(out)]) == 0
+    res = json.loads(out.read_text())[bench_runner.OP_PROGRAM_KEY]["datasets"]
+    assert set(res) == {f"OptionPricing-data/{n}.in" for n in ("small", "medium", "large")}
+    for v in res.values():
+        assert len(v["runtimes"]) == 2 and min(v["runtimes"]) > 0 and v["stderr"] == ""
