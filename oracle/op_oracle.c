@@ -16,8 +16,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted-mistake switch: non-zero values make one function of the oracle
+ * wrong on purpose, so the tests can show that their checks catch it. */
+static int op_oracle_mutation = 0;
+void op_oracle_set_mutation(int m) { op_oracle_mutation = m; }
+#define OPMUT(k) (op_oracle_mutation == (k))
+
 /* OptionPricing.fut:12 */
-static int32_t gray_code(int32_t x) { return (x >> 1) ^ x; }
+static int32_t gray_code(int32_t x) { return OPMUT(1) ? x : (x >> 1) ^ x; }
 
 /* OptionPricing.fut:17-27 (testBit, xorInds) for one direction vector. */
 static int32_t xor_inds(int32_t n, const int32_t *dir_v, int64_t num_bits) {
@@ -80,7 +86,8 @@ static const float kTailB[8] = {1.0f, 0.59983220655588793769f, 0.136929880922735
 float op_ugaussian(float p) {
   const float dp = p - 0.5f;
   if ((dp < 0.0f && 0.0f - dp <= 0.425f) || (0.0f <= dp && dp <= 0.425f))
-    return dp * poly_appr(0.180625f - dp * dp, kSmallA, kSmallB);
+    return dp * poly_appr(OPMUT(2) ? 0.180625f + dp * dp : 0.180625f - dp * dp, kSmallA,
+                          kSmallB);
   const float pp = dp < 0.0f ? dp + 0.5f : 0.5f - dp;
   const float r = sqrtf(-logf(pp));
   const float x = r <= 5.0f ? poly_appr(r - 1.6f, kMidA, kMidB)
@@ -100,7 +107,7 @@ static void bb_dates(int64_t D, const int32_t *bb_inds, const float *bb_data,
     const int32_t j = li[i] - 1, k = ri[i] - 1, l = bi[i] - 1;
     const float wk = row[k];
     const float zi = gauss[i * stride];
-    const float tmp = rw[i] * wk + sd[i] * zi;
+    const float tmp = (OPMUT(3) ? lw[i] : rw[i]) * wk + sd[i] * zi;
     row[l] = j == -1 ? tmp : tmp + lw[i] * row[j];
   }
   for (int64_t ii = 1; ii < D; ii++) {
@@ -131,7 +138,7 @@ static void black_scholes(int64_t D, int64_t U, const float *bb, const float *md
     float acc = 1.0f; /* the scan of (map2 (*)) from replicate 1.0 */
     for (int64_t d = 0; d < D; d++) {
       float noise = 0.0f; /* f32.sum of map2 (*) (take (u+1) zi) (take (u+1) md_c[u]) */
-      for (int64_t k = 0; k <= u; k++) noise += bb[d * U + k] * md_c[u * U + k];
+      for (int64_t k = 0; k <= (OPMUT(4) ? U - 1 : u); k++) noise += bb[d * U + k] * md_c[u * U + k];
       const float c = drifts[d * U + u] + noise * vols[d * U + u];
       acc = acc * expf(c);
       prices[d * U + u] = starts[u] * acc;
@@ -182,7 +189,7 @@ static float payoff(int32_t contract, const float *disc, const float *detval,
       cond |= x[0] <= 2630.6349999999998f || x[1] <= 8288.0f || x[2] <= 840.0f;
     }
     const float price1 = 100.0f * disc[0];
-    const float *last = xss + 3 * 366;
+    const float *last = xss + 3 * (OPMUT(5) ? 365 : 366);
     const int goto40 = cond && (last[0] < 3758.05f || last[1] < 11840.0f || last[2] < 1200.0f);
     const float amount = goto40 ? 1000.0f * fmin_payoff(last) : 1000.0f;
     const float price2 = amount * disc[1];

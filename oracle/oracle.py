@@ -250,6 +250,18 @@ def explicit_method(D, DD, mu, var, res) -> np.ndarray:
 # the Sobol construction against a brute-force Gray-code recurrence, and the
 # inverse Gaussian against the normal CDF (tests/test_oracle_op.py).
 
+def op_set_mutation(m: int) -> None:
+    """Plant mistake m (0 = none) in the OptionPricing oracle: 1 Gray code
+    dropped, 2 sign of dp^2 in the central polynomial, 3 left weight for the
+    right neighbour in the bridge, 4 the correlation summed over the whole
+    row instead of the lower triangle, 5 the
+    second-to-last date's prices in payoff3."""
+    L = lib()
+    L.op_oracle_set_mutation.argtypes = [ctypes.c_int]
+    L.op_oracle_set_mutation.restype = None
+    L.op_oracle_set_mutation(int(m))
+
+
 def op_sobol(n: int, dir_vs: np.ndarray) -> np.ndarray:
     """sobolIndR (:29-35) of path n over dir_vs [dims][bits]."""
     dv = np.ascontiguousarray(dir_vs, dtype=np.int32)

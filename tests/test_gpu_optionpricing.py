@@ -114,6 +114,17 @@ def test_31_direction_bits(ctx, datasets):
     np.testing.assert_allclose(got, oracle.op_main(args[0], n, dv, *args[3:]), rtol=RTOL)
 
 
+@pytest.mark.parametrize("contract", [1, 2, 3])
+def test_deterministic_paths_closed_form(ctx, contract):
+    """Zero volatilities: every path's prices follow the drifts, and the
+    payoff has a closed form (see test_op_oracle.deterministic_case)."""
+    from test_op_oracle import deterministic_case
+    args, want = deterministic_case(contract)
+    got = ctx.main(*args)
+    np.testing.assert_allclose(got, [want], rtol=1e-4)
+    np.testing.assert_allclose(got, oracle.op_main(*args), rtol=RTOL)
+
+
 def test_repeated_calls_identical(ctx, datasets):
     """The sum order is fixed by the path count: a result does not change
     from call to call."""
