@@ -16,9 +16,7 @@ extern "C" {
 /* Device pointers to the market data and the Brownian-bridge tables, as the
  * entry point received them (row-major), and the sizes. */
 typedef struct {
-  const int32_t *dv;   /* direction vectors [dims][OP_DV_STRIDE], zero past bits
-                          (filled by op_launch) */
-  int32_t bits;
+  int32_t bits;        /* direction-vector bits */
   const float *cs, *vols, *drifts, *sts, *detvals, *discts;
   int32_t K, M;        /* discounts per model, models */
   const int32_t *bbi;  /* [3][dates] */

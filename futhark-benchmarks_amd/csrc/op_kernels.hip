@@ -257,7 +257,13 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
     double *__restrict__ partial) {
   const Market mk = {cs, vols, drifts, sts, detvals, discts, K};
   constexpr int D = Shape<C>::D, U = Shape<C>::U, DIMS = D * U;
-  __shared__ int32_t xhi[kWaves][DIMS];
+  // The high part of the Sobol numbers: for contract 1's one dimension
+  // each wave makes its own from its wave-uniform Gray code (scalar loads
+  // and scalar ALU, no barrier); for more dimensions the workgroup
+  // tabulates it in LDS (per-wave scalar evaluation of contract 2's 15
+  // measured 1.9x slower).
+  constexpr bool kLdsHigh = DIMS > 1;
+  __shared__ int32_t xhi[kWaves][kLdsHigh ? DIMS : 1];
   __shared__ float tailbuf[kWaves][U * 64];
   __shared__ double red[kThreads];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -271,25 +277,35 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
   const int64_t nblk = N / kThreads + 1;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const int64_t n0 = blk * kThreads;
-    __syncthreads();  // the previous block's readers of xhi are done
-    for (int e = tid; e < kWaves * DIMS; e += kThreads) {
-      const int w = e / DIMS, d = e - w * DIMS;
-      const uint32_t c = (uint32_t)(n0 + 64 * w);
-      const uint32_t g = c ^ (c >> 1);  // grayCode (:12) of the wave's base
-      const int32_t *v = dv + d * OP_DV_STRIDE;
-      int32_t x = 0;
+    // grayCode (:12) of this wave's 64-aligned base (bits 0-4 are zero)
+    const uint32_t cw = (uint32_t)(n0 + 64 * wave);
+    const uint32_t ghi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cw ^ (cw >> 1)));
+    if constexpr (kLdsHigh) {
+      __syncthreads();  // the previous block's readers of xhi are done
+      for (int e = tid; e < kWaves * DIMS; e += kThreads) {
+        const int w = e / DIMS, d = e - w * DIMS;
+        const uint32_t c = (uint32_t)(n0 + 64 * w);
+        const uint32_t g = c ^ (c >> 1);
+        const int32_t *v = dv + d * OP_DV_STRIDE;
+        int32_t x = 0;
 #pragma unroll
-      for (int b = 0; b < OP_DV_STRIDE; ++b) x ^= ((g >> b) & 1u) ? v[b] : 0;
-      xhi[w][d] = x;
+        for (int b = 0; b < OP_DV_STRIDE; ++b) x ^= ((g >> b) & 1u) ? v[b] : 0;
+        xhi[w][d] = x;
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const int64_t n = n0 + tid;  // Sobol index of path n - 1 (:314)
     const bool live = n >= 1 && n <= N;
-    // the Gaussian of Sobol dimension d of this path (:29-35, :121-140)
     // the Sobol number of dimension d of this path (:29-35)
     auto sobol = [&](int d) {
       const int32_t *v = dv + d * OP_DV_STRIDE;
-      int32_t x = xhi[wave][d];
+      int32_t x = 0;
+      if constexpr (kLdsHigh) {
+        x = xhi[wave][d];
+      } else {
+#pragma unroll
+        for (int b = 5; b < OP_DV_STRIDE; ++b) x ^= ((ghi >> b) & 1u) ? v[b] : 0;
+      }
 #pragma unroll
       for (int b = 0; b < 6; ++b) x ^= ((glo >> b) & 1u) ? v[b] : 0;
       return (float)x / divisor;
