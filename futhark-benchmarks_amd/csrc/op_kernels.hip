@@ -271,7 +271,11 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
   double sum[OP_MAX_MODELS];
 #pragma unroll
   for (int m = 0; m < OP_MAX_MODELS; ++m) sum[m] = 0.0;
-  const float divisor = (float)(1u << bits);  // 2 ** num_bits (:31), bits <= 31
+  // x / 2 ** num_bits (:31-35): for any i32 x and bits <= 31 the quotient
+  // is a power-of-two scaling of (float)x with no underflow, hence exact,
+  // and so is the product with the exact reciprocal 2 ** -num_bits: same
+  // bits, one multiply instead of an IEEE division
+  const float inv_divisor = 1.0f / (float)(1u << bits);
   float rows[U][D];
   // blocks of kThreads Sobol indices covering 0 .. N (index 0 is no path)
   const int64_t nblk = N / kThreads + 1;
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
       }
 #pragma unroll
       for (int b = 0; b < 6; ++b) x ^= ((glo >> b) & 1u) ? v[b] : 0;
-      return (float)x / divisor;
+      return (float)x * inv_divisor;
     };
     // the Gaussians (:121-140) of dimensions i * U .. i * U + U - 1
     auto gauss_date = [&](int i, float(&z)[U]) {
