@@ -125,6 +125,40 @@ def test_deterministic_paths_closed_form(ctx, contract):
     np.testing.assert_allclose(got, oracle.op_main(*args), rtol=RTOL)
 
 
+def _random_bridge(D, rng, permutation=True, scale=0.1):
+    """bb_inds / bb_data for a bridge in a random order: step i writes date
+    bi[i] from an earlier-written right neighbour and, or not (li = 0, j =
+    -1), an earlier-written left one.  Without `permutation` some dates are
+    written twice and others never (they stay 0, :153)."""
+    bi = rng.permutation(D) + 1 if permutation else rng.integers(1, D + 1, D)
+    li = np.zeros(D, np.int64)
+    ri = np.zeros(D, np.int64)
+    for i in range(1, D):
+        ri[i] = bi[rng.integers(0, i)]
+        li[i] = 0 if rng.random() < 0.3 else bi[rng.integers(0, i)]
+    ri[0] = li[0] = 1
+    inds = np.stack([bi, li, ri]).astype(np.int32)
+    data = (rng.random((3, D)) * np.array([[scale], [0.5], [0.5]])).astype(np.float32)
+    return inds, data
+
+
+@pytest.mark.parametrize("name", ["medium", "large"])
+@pytest.mark.parametrize("permutation", [True, False])
+def test_random_bridge_tables(ctx, datasets, name, permutation):
+    """The generic bridge indexing (:153-177) beyond the datasets' tables."""
+    args, _ = datasets[name]
+    rng = np.random.default_rng(11 if permutation else 12)
+    D = args[9].shape[1]
+    # large: wide enough paths that the barriers of payoff3 are reached
+    bbi, bbd = _random_bridge(D, rng, permutation, scale=0.1 if name == "medium" else 4.0)
+    a = args[:9] + (bbi, bbd)
+    n = 3000
+    got = ctx.main(a[0], n, *a[2:])
+    want = oracle.op_main(a[0], n, *a[2:])
+    assert np.all(np.isfinite(want))
+    np.testing.assert_allclose(got, want, rtol=RTOL)
+
+
 def test_repeated_calls_identical(ctx, datasets):
     """The sum order is fixed by the path count: a result does not change
     from call to call."""
