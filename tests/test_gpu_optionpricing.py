@@ -159,6 +159,18 @@ def test_random_bridge_tables(ctx, datasets, name, permutation):
     np.testing.assert_allclose(got, want, rtol=RTOL)
 
 
+def test_profiling_report(datasets):
+    """With profiling on, futhark_context_report counts the entry's kernel
+    launches and their time (JSON, as the generated API's report)."""
+    import json
+    args, _ = datasets["medium"]
+    with op.OptionPricing(device=0, profiling=True) as c:
+        c.main(args[0], 5000, *args[2:])
+        c.main(args[0], 5000, *args[2:])
+        rep = json.loads(c.report())
+    assert rep["op_paths"]["count"] == 2 and rep["op_paths"]["total_us"] > 0
+
+
 def test_repeated_calls_identical(ctx, datasets):
     """The sum order is fixed by the path count: a result does not change
     from call to call."""
