@@ -8,11 +8,47 @@
  * sweep once for the whole batch; a problem whose numT-1 steps are done
  * drops out (its workgroups return at once) until the longest has finished.
  */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "../../include/locvolcalib.h"
 #include "lvc_runtime.h"
+
+/* i32.f32 / i32.f64 (truncation toward zero) of a value in the i32 range;
+ * anything else (NaN, infinities, out of range) gives INT32_MIN, which lies
+ * outside every grid. */
+static int32_t trunc_i32(double v) {
+  if (!(v > -2147483649.0 && v < 2147483648.0))
+    return INT32_MIN;
+  return (int32_t)v;
+}
+
+/* LocVolCalib.fut:15-20.  The f32 branch performs the reference's own f32
+ * operations in its order (C on this host evaluates float expressions in
+ * single precision, IEEE-rounded, without contraction: sqrtf is correctly
+ * rounded), so myXindex and x[0] are the reference's bits. */
+void lvc_grid_scalars(lvc_problem *q, int grid32) {
+  q->grid32 = grid32;
+  q->yindex = q->ny / 2; /* i32.i64 numY / 2 */
+  q->dy = 10.0 * q->nu * sqrt(q->t) / (double)q->ny;
+  if (grid32) {
+    const float s0 = (float)q->s0, t = (float)q->t, alpha = (float)q->alpha;
+    const float stdX = 20.0f * alpha * s0 * sqrtf(t);
+    const float dx = stdX / (float)q->nx;
+    q->xindex = trunc_i32(s0 / dx);
+    /* myX[0] = f32.i64 0 * dx - f32.i32 myXindex * dx + s0 (:21) */
+    const float x0 = 0.0f * dx - (float)q->xindex * dx + s0;
+    q->dx = dx;
+    q->x0log = x0;
+  } else {
+    const double stdX = 20.0 * q->alpha * q->s0 * sqrt(q->t);
+    q->dx = stdX / (double)q->nx;
+    q->xindex = trunc_i32(q->s0 / q->dx);
+    q->x0log = 0.0 * q->dx - (double)q->xindex * q->dx + q->s0;
+  }
+  q->status = (q->xindex >= 0 && q->xindex < q->nx) ? 0 : 3;
+}
 
 int lvc_validate_sizes(struct futhark_context *ctx, int64_t nx, int64_t ny,
                        int64_t nt) {
@@ -111,7 +147,8 @@ static int batch_xshare(struct futhark_context *ctx, int precision, int nruns,
  * (LocVolCalib.fut:244-251). */
 static int same_factor(const lvc_problem *a, const lvc_problem *b) {
   return a->nx == b->nx && a->ny == b->ny && a->nt == b->nt && a->s0 == b->s0 &&
-         a->t == b->t && a->alpha == b->alpha && a->nu == b->nu && a->beta == b->beta;
+         a->t == b->t && a->alpha == b->alpha && a->nu == b->nu && a->beta == b->beta &&
+         a->grid32 == b->grid32;
 }
 
 int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
@@ -148,7 +185,7 @@ int lvc_batch_build(struct futhark_context *ctx, lvc_batch *b, int n,
     q.geo_off = geo;
     q.state_off = state;
     q.k = -1;
-    q.status = 0;
+    lvc_grid_scalars(&q, q.grid32);
     q.ydone = 0;
     q.fac_src = p;
     q.ytiles = (q.nx + cols - 1) / cols;
@@ -372,6 +409,9 @@ int lvc_batch_gather(struct futhark_context *ctx, lvc_batch *b, void *out,
   return lvc_check_hip(ctx, e, "gather kernel");
 }
 
+static int download(struct futhark_context *ctx, void *dst, const void *src,
+                    size_t bytes);
+
 /* Runs a batch end to end into a new device array of n prices. */
 static int run_batch(struct futhark_context *ctx, int n, const lvc_problem *probs,
                      int precision, int out_f64, void **out_mem) {
@@ -393,6 +433,16 @@ static int run_batch(struct futhark_context *ctx, int n, const lvc_problem *prob
     rc = lvc_batch_run(ctx, &b);
   if (!rc)
     rc = lvc_batch_gather(ctx, &b, *out_mem, out_f64);
+  if (!rc && n > 0 && ctx->cfg && ctx->cfg->debugging) {
+    /* the index the gather read: problem 0's device descriptor */
+    lvc_problem q;
+    rc = download(ctx, &q, b.d_probs, sizeof(q));
+    if (!rc) {
+      ctx->dbg_index[0] = q.yindex;
+      ctx->dbg_index[1] = q.xindex;
+      ctx->dbg_index_valid = 1;
+    }
+  }
   lvc_batch_release(ctx, &b);
   if (rc) {
     lvc_dev_free(ctx, *out_mem, ob);
@@ -402,7 +452,8 @@ static int run_batch(struct futhark_context *ctx, int n, const lvc_problem *prob
 }
 
 static void fill_main(lvc_problem *probs, int outer, int nx, int ny, int nt,
-                      double s0, double t, double alpha, double nu, double beta) {
+                      double s0, double t, double alpha, double nu, double beta,
+                      int grid32) {
   memset(probs, 0, (size_t)outer * sizeof(*probs));
   for (int i = 0; i < outer; i++) {
     probs[i].nx = nx;
@@ -413,7 +464,33 @@ static void fill_main(lvc_problem *probs, int outer, int nx, int ny, int nt,
     probs[i].alpha = alpha;
     probs[i].nu = nu;
     probs[i].beta = beta;
+    probs[i].grid32 = grid32;
   }
+}
+
+/* The price index of main's options (all share initGrid's scalars): the
+ * reference's `myResult[myYindex, myXindex]` (:236) is bounds-checked, so an
+ * index outside the grid fails the call, as the generated code does, before
+ * any device work. */
+static int check_main_index(struct futhark_context *ctx, int outer, int nx, int ny,
+                            double s0, double t, double alpha, double nu,
+                            int grid32) {
+  if (outer == 0)
+    return 0;
+  lvc_problem q;
+  memset(&q, 0, sizeof(q));
+  q.nx = nx;
+  q.ny = ny;
+  q.s0 = s0;
+  q.t = t;
+  q.alpha = alpha;
+  q.nu = nu;
+  lvc_grid_scalars(&q, grid32);
+  if (q.status == 0)
+    return 0;
+  lvc_set_error(ctx, "Index [%d, %d] out of bounds for array of shape [%d][%d].",
+                q.yindex, q.xindex, ny, nx);
+  return 1;
 }
 
 /* LocVolCalib.fut:238-252 */
@@ -429,14 +506,16 @@ int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0
     lvc_set_error(ctx, "outer_loop_count must be non-negative (got %d)", outer);
     return 1;
   }
-  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx))
+  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx) ||
+      check_main_index(ctx, outer, numX, numY, s0, t, alpha, nu, 1))
     return 1;
   lvc_problem *probs = malloc((size_t)(outer ? outer : 1) * sizeof(*probs));
   if (!probs) {
     lvc_set_error(ctx, "host allocation failed");
     return 1;
   }
-  fill_main(probs, outer, numX, numY, numT, s0, t, alpha, nu, beta);
+  /* the f32 entry: initGrid's scalars in f32, as the reference */
+  fill_main(probs, outer, numX, numY, numT, s0, t, alpha, nu, beta, 1);
   for (int i = 0; i < outer; i++)
     probs[i].strike = (double)(0.001f * (float)(int64_t)i); /* :247, in f32 */
   pthread_mutex_lock(&ctx->lock);
@@ -468,14 +547,15 @@ int futhark_entry_main_f64(struct futhark_context *ctx,
     lvc_set_error(ctx, "outer_loop_count must be non-negative (got %d)", outer);
     return 1;
   }
-  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx))
+  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx) ||
+      check_main_index(ctx, outer, numX, numY, s0, t, alpha, nu, 0))
     return 1;
   lvc_problem *probs = malloc((size_t)(outer ? outer : 1) * sizeof(*probs));
   if (!probs) {
     lvc_set_error(ctx, "host allocation failed");
     return 1;
   }
-  fill_main(probs, outer, numX, numY, numT, s0, t, alpha, nu, beta);
+  fill_main(probs, outer, numX, numY, numT, s0, t, alpha, nu, beta, 0);
   for (int i = 0; i < outer; i++)
     probs[i].strike = 0.001 * (double)(int64_t)i;
   pthread_mutex_lock(&ctx->lock);

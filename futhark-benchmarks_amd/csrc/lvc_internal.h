@@ -36,16 +36,21 @@ enum lvc_precision {
 
 typedef struct lvc_problem {
   int32_t nx, ny, nt;
-  int32_t xindex, yindex;  /* written by the setup kernel */
+  int32_t xindex, yindex;  /* initGrid's price indexes (lvc_grid_scalars) */
   int32_t k;               /* next time index to process; < 0 when done */
   int32_t status;          /* 0 = ok, 3 = price index out of range */
   int32_t ytiles;          /* y-sweep workgroups of this problem per step */
   int32_t ydone;           /* y-sweep workgroups finished in this step */
   int32_t fac_src;         /* problem whose y factor table this one uses */
-  int32_t pad_[3];
+  int32_t grid32;          /* initGrid's scalars in f32 (the reference's f32 entry) */
+  int32_t pad_[2];
   int64_t state_off;       /* element offset of R and Y */
   int64_t geo_off;         /* element offset into the f64 geometry buffer */
   double s0, strike, t, alpha, nu, beta;
+  /* initGrid's scalars (LocVolCalib.fut:15-20), set on the host by
+   * lvc_grid_scalars: the spacings and the value of x[0] whose log VarX
+   * takes (:63).  With grid32 they are the reference's f32 values, widened. */
+  double dx, dy, x0log;
 } lvc_problem;
 
 /* One workgroup's share of a sweep: `count` rows (x sweep) or columns (y
@@ -72,6 +77,15 @@ typedef struct lvc_tile {
 /* real elements of one problem's y factor table (column y sweep) */
 #define LVC_YFAC_LEN (6 * LVC_COL_ROWS + 6 * (LVC_COL_BLOCK / 64))
 #define LVC_MAX_N 1024 /* largest numX / numY accepted (64 lanes x 16) */
+
+/* initGrid's scalars of one problem (LocVolCalib.fut:15-20) into q->dx, dy,
+ * x0log, xindex, yindex and q->status (3 when myXindex is outside the grid,
+ * the reference's failing bounds check at :236).  grid32 = 1 evaluates
+ * stdX = 20*alpha*s0*sqrt(t), dx = stdX/numX, myXindex = i32.f32(s0/dx) and
+ * x[0] in f32, as the reference's f32 program does (the parameters are the
+ * f32 values the entry received); grid32 = 0 in FP64.  The grid arrays are
+ * then built in FP64 from these scalars by the setup kernel. */
+void lvc_grid_scalars(lvc_problem *q, int grid32);
 
 /* Supported segment lengths (elements of one system per lane): 4, 8, 16 for
  * both sweeps (the lanes of a system are lanes of one wavefront, reduced

@@ -255,11 +255,16 @@ struct Taps {
 };
 
 // ---------------------------------------------------------------------------
-// Setup: initGrid (:12-23), initOperator (:26-44) taps, the VarX factor
-// x^(2 beta) of updateParams (:61-65) and setPayoff (:46-47).  Computed in
+// Setup: initGrid's arrays (:12-23), initOperator (:26-44) taps, the VarX
+// factor x^(2 beta) of updateParams (:61-65) and setPayoff (:46-47).  The
+// scalars (dx, dy, myXindex, myYindex: :15-20) come from the host in the
+// descriptor, in the entry's own precision; the arrays are computed here in
 // FP64 with multiply-add contraction disabled, so the grid is bit-identical
-// to a C restatement of the same FP64 lines.  Grid: (problems, row chunks);
-// chunk 0 also writes the geometry and the descriptor fields.
+// to a C restatement of the same FP64 lines.  VarX's log at column 0 takes
+// the descriptor's x0log (the reference's own f32 x[0] for the f32 entry):
+// it decides whether that log is finite, -inf (x[0] = 0) or NaN (x[0] < 0).
+// Grid: (problems, row chunks); chunk 0 also writes the geometry and resets
+// the time index.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
@@ -267,10 +272,9 @@ __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
 #pragma clang fp contract(off)
   lvc_problem &pr = probs[blockIdx.x];
   const int nx = pr.nx, ny = pr.ny;
-  const double s0 = pr.s0, t = pr.t, alpha = pr.alpha, nu = pr.nu, beta = pr.beta;
-  const double stdX = 20.0 * alpha * s0 * sqrt(t);
-  const double dx = stdX / (double)nx;
-  const int xi = (int)(s0 / dx);  // i32.f64: truncation toward zero
+  const double s0 = pr.s0, alpha = pr.alpha, nu = pr.nu, beta = pr.beta;
+  const double dx = pr.dx, dy = pr.dy;
+  const int xi = pr.xindex, yi = pr.yindex;
 
   // payoff rows of this chunk: R[j][i] = max(x_i - K, 0)
   T *res = R + pr.state_off;
@@ -285,9 +289,6 @@ __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
   if (blockIdx.y != 0) return;
 
   const double logAlpha = log(alpha);
-  const double stdY = 10.0 * nu * sqrt(t);
-  const double dy = stdY / (double)ny;
-  const int yi = ny / 2;
   double *x = geo + pr.geo_off;
   double *ax = x + nx, *cx = ax + nx, *y = cx + nx, *ay = y + ny, *cy = ay + ny;
   float *f32 = reinterpret_cast<float *>(geo + pr.geo_off + LVC_GEO_F64(nx, ny));
@@ -297,12 +298,7 @@ __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
     x[i] = (double)i * dx - (double)xi * dx + s0;
   for (int j = threadIdx.x; j < ny; j += blockDim.x)
     y[j] = (double)j * dy - (double)yi * dy + logAlpha;
-  if (threadIdx.x == 0) {
-    pr.xindex = xi;
-    pr.yindex = yi;
-    pr.k = pr.nt - 2;
-    pr.status = (xi >= 0 && xi < nx) ? 0 : 3;
-  }
+  if (threadIdx.x == 0) pr.k = pr.nt - 2;
   __syncthreads();
   for (int i = threadIdx.x; i < LVC_P4(nx); i += blockDim.x) {
     double d0 = 0.0, d2 = 0.0;
@@ -312,7 +308,7 @@ __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
       d2 = 2.0 / du / (dl + du);
     }
     if (i < nx) {
-      const double X = exp(2.0 * (beta * log(x[i])));
+      const double X = exp(2.0 * (beta * log(i == 0 ? pr.x0log : x[i])));
       ax[i] = -0.25 * X * d0;
       cx[i] = -0.25 * X * d2;
     }

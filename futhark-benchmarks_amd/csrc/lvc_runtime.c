@@ -433,7 +433,7 @@ char *futhark_context_report(struct futhark_context *ctx) {
       lvc_prof_collect(ctx);
     pthread_mutex_unlock(&ctx->lock);
   }
-  size_t cap = 256 + 128 * (size_t)ctx->nprof;
+  size_t cap = 320 + 128 * (size_t)ctx->nprof;
   char *out = malloc(cap);
   if (!out)
     return NULL;
@@ -443,6 +443,9 @@ char *futhark_context_report(struct futhark_context *ctx) {
                             "%s\"%s\": {\"count\": %lld, \"total_us\": %.3f}",
                             i ? ", " : "", ctx->prof[i].name,
                             (long long)ctx->prof[i].count, ctx->prof[i].total_us);
+  if (ctx->dbg_index_valid)
+    len += (size_t)snprintf(out + len, cap - len, "%s\"price_index\": [%d, %d]",
+                            ctx->nprof ? ", " : "", ctx->dbg_index[0], ctx->dbg_index[1]);
   snprintf(out + len, cap - len, "}");
   return out;
 }

@@ -80,6 +80,10 @@ struct futhark_context {
   lvc_tuned tuned[LVC_MAX_TUNED];
   int num_cus; /* compute units of the device */
   int ntuned;
+  /* debugging: the price index (myYindex, myXindex) the last entry call's
+   * gather read, from the device descriptor of its first problem */
+  int dbg_index_valid;
+  int32_t dbg_index[2];
 };
 
 /* What differs between the programs built on this runtime: the tuning

@@ -294,6 +294,7 @@ static int admit(struct lvc_server *s) {
     q.nu = qr->req.nu;
     q.beta = qr->req.beta;
     q.k = -1;
+    lvc_grid_scalars(&q, 0);
     q.state_off = (int64_t)slot * s->slot_elems;
     q.geo_off = (int64_t)slot * s->slot_geo;
     q.fac_src = slot;

@@ -172,7 +172,8 @@ class LocVolCalib:
 
     def __init__(self, device: int = 0, precision: Optional[int] = None,
                  profiling: bool = False, cache_file: Optional[str] = None,
-                 tuning: Optional[dict] = None, logging: bool = False):
+                 tuning: Optional[dict] = None, logging: bool = False,
+                 debugging: bool = False):
         L = lib()
         self._L = L
         self._cfg = L.futhark_context_config_new()
@@ -183,6 +184,7 @@ class LocVolCalib:
         L.futhark_context_config_set_device(self._cfg, str(device).encode())
         L.futhark_context_config_set_profiling(self._cfg, int(profiling))
         L.futhark_context_config_set_logging(self._cfg, int(logging))
+        L.futhark_context_config_set_debugging(self._cfg, int(debugging))
         if cache_file:
             L.futhark_context_config_set_cache_file(self._cfg, cache_file.encode())
         params = dict(tuning or {})
@@ -239,6 +241,13 @@ class LocVolCalib:
 
     def report(self) -> str:
         return _take_string(self._L.futhark_context_report(self._ctx)) or "{}"
+
+    def price_index(self) -> Optional[tuple]:
+        """(myYindex, myXindex) that the last entry call's price gather read
+        from the device (LocVolCalib.fut:20, :236); needs ``debugging``."""
+        import json
+        idx = json.loads(self.report()).get("price_index")
+        return tuple(idx) if idx is not None else None
 
     def pause_profiling(self) -> None:
         self._L.futhark_context_pause_profiling(self._ctx)

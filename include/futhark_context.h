@@ -56,14 +56,25 @@ void futhark_context_config_set_cache_file(struct futhark_context_config *cfg,
  *       results (the reference's version-selection thresholds);
  *   "lvc.precision"  0 = FP32, 1 = FP32 + one FP64-residual refinement per
  *       implicit solve (default), 2 = FP64 (for futhark_entry_main);
- *   "lvc.seg_x", "lvc.seg_y"  segment length of the x / y sweeps (2..32,
- *       power of two; 0 = autotuned);
- *   "lvc.autotune"  1 = time the candidate segment lengths on first use of
- *       a size class (default 0: built-in table or cache file).
- * Returns non-zero for an unknown name. */
+ *   "lvc.seg_x"  unknowns per lane of the x sweep's row systems: 4, 8 or
+ *       16 (0 = tuned per size class);
+ *   "lvc.seg_y"  the y sweep's solver: lane segments 4, 8, 16 (PCR across
+ *       the lanes of a column) or column segments 32, 64 (one lane per
+ *       column segment, Thomas; 0 = tuned per size class);
+ *   "lvc.autotune"  1 = time the candidates on first use of a size class
+ *       (default 0: the cache file, else the built-in per-size table);
+ *   "lvc.xshare"  options per x tile sharing one row factorisation, 1..16
+ *       (0 = chosen per batch);
+ *   "lvc.graph"  1 = replay a batch's step launches from a HIP graph kept
+ *       for batches of the same shape (default 0: direct launches).
+ * A size too large for the chosen segment length takes the next one that
+ * fits.  Returns non-zero for an unknown name or an unsupported value. */
 int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,
                                             const char *param_name,
                                             size_t new_value);
+/* Debugging: each entry call also reads back, after its price gather, the
+ * price index (myYindex, myXindex) its first option used, and
+ * futhark_context_report lists it as "price_index". */
 void futhark_context_config_set_debugging(struct futhark_context_config *cfg,
                                           int flag);
 /* Per-kernel timing with HIP events; read back with futhark_context_report. */

@@ -29,10 +29,15 @@ extern "C" {
 
 /* LocVolCalib.fut:238-252 (`main`): prices of `outer` options with strikes
  * 0.001*i, i = 0..outer-1, all sharing the grid sizes and parameters.
- * Computed in the precision set by "lvc.precision" (default FP32 with one
- * FP64-residual refinement per implicit solve).  Errors: numX or numY < 3 or
- * numT < 2 (the reference's range `1...n-2` at :37 and its timeline), outer
- * < 0 (`iota` at :247), device failure.  outer == 0 returns an empty array. */
+ * initGrid's scalars (stdX, dx, myXindex = i32.f32(s0/dx): :15-20) are
+ * formed in f32 from the f32 arguments, as the reference does, so the price
+ * index is the reference's; the rest is computed in the precision set by
+ * "lvc.precision" (default FP32 with one FP64-residual refinement per
+ * implicit solve, on an FP64 grid).  Errors: numX or numY < 3 or numT < 2
+ * (the reference's range `1...n-2` at :37 and its timeline), outer < 0
+ * (`iota` at :247), a price index outside the grid (the bounds check of
+ * `myResult[myYindex, myXindex]` at :236, e.g. 20*alpha*sqrt(t) <= 1),
+ * device failure.  outer == 0 returns an empty array. */
 int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0,
                        const int32_t outer_loop_count, const int32_t numX,
                        const int32_t numY, const int32_t numT, const float s0,
@@ -40,7 +45,8 @@ int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0
                        const float beta);
 
 /* The same program with every real f64 (the FP64 plain form): the f32 ->
- * f64 type substitution of LocVolCalib.fut:238-252.  No counterpart in the
+ * f64 type substitution of LocVolCalib.fut:238-252, initGrid's scalars
+ * included.  Errors as futhark_entry_main.  No counterpart in the
  * reference, which is f32-only; named as the generator names an extra
  * `entry main_f64`. */
 int futhark_entry_main_f64(struct futhark_context *ctx,
@@ -53,9 +59,11 @@ int futhark_entry_main_f64(struct futhark_context *ctx,
 /* Batched form: problem p has its own sizes (numX[p], numY[p], numT[p]) and
  * parameters and runs LocVolCalib.fut:211-236 (`value`).  All arrays have
  * the same length n.  `precision` as "lvc.precision".  The problems advance
- * together, one launch of each sweep per time step (replayed from a HIP
- * graph), until each has done its own numT-1 steps.  No counterpart in the
- * reference (its `main` shares one size and parameter set). */
+ * together, one launch of each sweep per time step over the problems still
+ * running (replayed from a HIP graph with "lvc.graph"), until each has done
+ * its own numT-1 steps.  A problem whose price index lies outside its grid
+ * gives NaN (the call succeeds).  No counterpart in the reference (its
+ * `main` shares one size and parameter set). */
 int futhark_entry_main_batch(struct futhark_context *ctx,
                              struct futhark_f64_1d **out0,
                              const struct futhark_i32_1d *numX,

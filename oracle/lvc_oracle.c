@@ -26,6 +26,11 @@ void lvc_oracle_set_mutation(int m) { lvc_oracle_mutation = m; }
 int lvc_oracle_get_mutation(void) { return lvc_oracle_mutation; }
 #define MUT(k) (lvc_oracle_mutation == (k))
 
+/* GRID32 switch: initGrid's scalars in f32 (see init_grid_ in the body). */
+static int lvc_oracle_grid32 = 0;
+void lvc_oracle_set_grid32(int on) { lvc_oracle_grid32 = on != 0; }
+#define GRID32 (lvc_oracle_grid32 != 0)
+
 #define R float
 #define SUF f32
 #define LOG logf

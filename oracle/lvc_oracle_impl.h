@@ -18,10 +18,17 @@
 
 /* LocVolCalib.fut:12-23 (initGrid).  x[myXindex] == s0 exactly when the
  * compiler does not contract `i*dx - xi*dx` (the oracle is built with
- * -ffp-contract=off). */
-void FN(lvc_init_grid_)(R s0, R alpha, R nu, R t, int64_t numX, int64_t numY,
-                        int64_t numT, int32_t *xindex, int32_t *yindex, R *myX,
-                        R *myY, R *timeline) {
+ * -ffp-contract=off).  *x0log receives the value whose log updateParams takes
+ * at column 0 (x[0] itself, but see GRID32 below).
+ *
+ * GRID32 mode (lvc_oracle_set_grid32): the scalars stdX, dx, myXindex and
+ * x[0] of :15-21 are evaluated in f32 from the f32 values of the parameters,
+ * as the reference's f32 program does, and the arrays in R from those
+ * scalars.  With R = double this is the GPU library's form of the reference's
+ * f32 entry (f32 grid index, FP64 grid); with R = float it changes nothing. */
+static void FN(init_grid_)(R s0, R alpha, R nu, R t, int64_t numX, int64_t numY,
+                           int64_t numT, int32_t *xindex, int32_t *yindex, R *myX,
+                           R *myY, R *timeline, R *x0log) {
   R logAlpha = LOG(alpha);
   for (int64_t i = 0; i < numT; i++)
     timeline[i] = t * (R)i / ((R)numT - (R)1.0);
@@ -30,12 +37,30 @@ void FN(lvc_init_grid_)(R s0, R alpha, R nu, R t, int64_t numX, int64_t numY,
   R dx = stdX / (R)numX, dy = stdY / (R)numY;
   int32_t xi = (int32_t)(s0 / dx); /* i32.f32 truncates toward zero */
   int32_t yi = (int32_t)numY / 2;
+  R x0 = (R)0 * dx - (R)xi * dx + s0;
+  if (GRID32) {
+    const float s0f = (float)s0, af = (float)alpha, tf = (float)t;
+    const float stdXf = 20.0f * af * s0f * sqrtf(tf);
+    const float dxf = stdXf / (float)numX;
+    xi = (int32_t)(s0f / dxf);
+    dx = (R)dxf;
+    x0 = (R)(0.0f * dxf - (float)xi * dxf + s0f);
+  }
   for (int64_t i = 0; i < numX; i++)
     myX[i] = (R)i * dx - (R)xi * dx + s0;
   for (int64_t i = 0; i < numY; i++)
     myY[i] = (R)i * dy - (R)yi * dy + logAlpha;
   *xindex = xi;
   *yindex = yi;
+  *x0log = x0;
+}
+
+void FN(lvc_init_grid_)(R s0, R alpha, R nu, R t, int64_t numX, int64_t numY,
+                        int64_t numT, int32_t *xindex, int32_t *yindex, R *myX,
+                        R *myY, R *timeline) {
+  R x0;
+  FN(init_grid_)(s0, alpha, nu, t, numX, numY, numT, xindex, yindex, myX, myY,
+                 timeline, &x0);
 }
 
 /* LocVolCalib.fut:26-44 (initOperator): D and DD are [n][3] row-major. */
@@ -75,14 +100,21 @@ void FN(lvc_set_payoff_)(R strike, const R *myX, int64_t numX, int64_t numY,
 }
 
 /* LocVolCalib.fut:50-66 (updateParams).  Only VarX is non-trivial:
- * MuX = MuY = 0 and VarY = nu*nu everywhere.  varX is [numY][numX]. */
-void FN(lvc_update_params_)(const R *myX, const R *myY, int64_t numX,
-                            int64_t numY, R tnow, R beta, R nu, R *varX) {
+ * MuX = MuY = 0 and VarY = nu*nu everywhere.  varX is [numY][numX]; the log
+ * at column 0 takes x0log (init_grid_). */
+static void FN(update_params_)(const R *myX, const R *myY, int64_t numX,
+                               int64_t numY, R tnow, R beta, R nu, R x0log,
+                               R *varX) {
   for (int64_t j = 0; j < numY; j++)
     for (int64_t i = 0; i < numX; i++)
       varX[j * numX + i] =
-          EXP((R)2.0 * (beta * LOG(myX[i]) + myY[j] -
+          EXP((R)2.0 * (beta * LOG(i == 0 ? x0log : myX[i]) + myY[j] -
                         (MUT(4) ? (R)-0.5 : (R)0.5) * nu * nu * tnow));
+}
+
+void FN(lvc_update_params_)(const R *myX, const R *myY, int64_t numX,
+                            int64_t numY, R tnow, R beta, R nu, R *varX) {
+  FN(update_params_)(myX, myY, numX, numY, tnow, beta, nu, myX[0], varX);
 }
 
 /* LocVolCalib.fut:68-124 (tridagPar): the three scans evaluated sequentially
@@ -352,8 +384,9 @@ R FN(lvc_value_)(int64_t numX, int64_t numY, int64_t numT, R s0, R strike,
     abort(); /* layout and allocation out of step */
 
   int32_t xi, yi;
-  FN(lvc_init_grid_)(s0, alpha, nu, t, numX, numY, numT, &xi, &yi, w.myX,
-                     w.myY, w.timeline);
+  R x0log;
+  FN(init_grid_)(s0, alpha, nu, t, numX, numY, numT, &xi, &yi, w.myX, w.myY,
+                 w.timeline, &x0log);
   FN(lvc_init_operator_)(w.myX, numX, w.Dx, w.Dxx);
   FN(lvc_init_operator_)(w.myY, numY, w.Dy, w.Dyy);
 
@@ -364,7 +397,7 @@ R FN(lvc_value_)(int64_t numX, int64_t numY, int64_t numT, R s0, R strike,
   /* Timeline neighbours, latest interval first (:217-219). */
   for (int64_t k = numT - 2; k >= 0; k--) {
     R tnow = w.timeline[k], tnext = w.timeline[k + 1];
-    FN(lvc_update_params_)(w.myX, w.myY, numX, numY, tnow, beta, nu, w.varX);
+    FN(update_params_)(w.myX, w.myY, numX, numY, tnow, beta, nu, x0log, w.varX);
     FN(rollback_)(tnow, tnext, numX, numY, &w, thomas);
   }
   R out;

@@ -13,6 +13,11 @@ oracle function        reference                              pinned by
 =====================  =====================================  ==========================================
 init_grid              LocVolCalib.fut:12-23                  closed forms: x[xindex] == s0, y[NY/2] ==
                                                               log(alpha), uniform spacing, timeline ends
+init_grid, grid32      :15-21 scalars in f32, arrays in f64   myXindex == the f32 restatement's index;
+                                                              x[xindex] == s0; f64 prices vs the f32
+                                                              restatement at the fixtures' tolerance;
+                                                              closed form s0 - K where the f32 index is
+                                                              integral (t=4, alpha=0.2)
 init_operator          LocVolCalib.fut:26-44                  exact identities: D*1 = 0, D*x = 1,
                                                               DD*x = 0, DD*x^2 = 2, boundary literals
 set_payoff             LocVolCalib.fut:46-47                  direct formula
@@ -97,6 +102,8 @@ def lib() -> ctypes.CDLL:
             f.argtypes = [_ptr] * 5 + [_c_i64, _c_i64, _ptr]
         _lib.lvc_oracle_set_mutation.argtypes = [ctypes.c_int]
         _lib.lvc_oracle_set_mutation.restype = None
+        _lib.lvc_oracle_set_grid32.argtypes = [ctypes.c_int]
+        _lib.lvc_oracle_set_grid32.restype = None
     return _lib
 
 
@@ -118,35 +125,57 @@ def set_mutation(m: int) -> None:
     lib().lvc_oracle_set_mutation(int(m))
 
 
+class _Grid32:
+    """initGrid's scalars (stdX, dx, myXindex, x[0]; LocVolCalib.fut:15-21)
+    in f32 for the duration of a call: with dtype float64 the GPU library's
+    form of the reference's f32 entry (the f32 grid index, an FP64 grid)."""
+
+    def __init__(self, on):
+        self.on = bool(on)
+
+    def __enter__(self):
+        if self.on:
+            lib().lvc_oracle_set_grid32(1)
+
+    def __exit__(self, *exc):
+        if self.on:
+            lib().lvc_oracle_set_grid32(0)
+
+
 def main(outer, numX, numY, numT, s0, t, alpha, nu, beta, dtype=np.float32,
-         thomas=False, nthreads=1) -> np.ndarray:
-    """LocVolCalib.fut:238-252: prices of ``outer`` options, strike 0.001*i."""
+         thomas=False, nthreads=1, grid32=False) -> np.ndarray:
+    """LocVolCalib.fut:238-252: prices of ``outer`` options, strike 0.001*i
+    (formed in ``dtype``).  ``grid32``: initGrid's scalars in f32."""
     suf, real = _suffix(dtype)
     out = np.zeros(max(int(outer), 0), dtype=real)
-    rc = getattr(lib(), "lvc_main" + suf)(outer, numX, numY, numT, s0, t, alpha,
-                                          nu, beta, int(thomas), int(nthreads),
-                                          _p(out) if out.size else None)
+    with _Grid32(grid32):
+        rc = getattr(lib(), "lvc_main" + suf)(outer, numX, numY, numT, s0, t, alpha,
+                                              nu, beta, int(thomas), int(nthreads),
+                                              _p(out) if out.size else None)
     if rc != 0:
         raise ValueError(f"oracle main failed with code {rc}")
     return out
 
 
 def value(numX, numY, numT, s0, strike, t, alpha, nu, beta, dtype=np.float64,
-          thomas=False) -> float:
+          thomas=False, grid32=False) -> float:
     """LocVolCalib.fut:211-236 for one option."""
     suf, real = _suffix(dtype)
     err = ctypes.c_int(0)
-    v = getattr(lib(), "lvc_value" + suf)(numX, numY, numT, s0, strike, t,
-                                           alpha, nu, beta, int(thomas),
-                                           ctypes.addressof(err))
+    with _Grid32(grid32):
+        v = getattr(lib(), "lvc_value" + suf)(numX, numY, numT, s0, strike, t,
+                                               alpha, nu, beta, int(thomas),
+                                               ctypes.addressof(err))
     if err.value:
         raise ValueError(f"oracle value failed with code {err.value}")
     return float(v)
 
 
 def batch(numX, numY, numT, s0, strike, t, alpha, nu, beta, dtype=np.float64,
-          thomas=False, nthreads=1) -> np.ndarray:
-    """Per-problem sizes and parameters; same arithmetic as ``value``."""
+          thomas=False, nthreads=1, grid32=False, allow_nan=False) -> np.ndarray:
+    """Per-problem sizes and parameters; same arithmetic as ``value``.  With
+    ``allow_nan`` a problem whose price index lies outside its grid gives NaN
+    instead of raising (the batch entry's convention)."""
     suf, real = _suffix(dtype)
     n = len(numX)
     ints = [np.ascontiguousarray(v, dtype=np.int32) for v in (numX, numY, numT)]
@@ -154,15 +183,16 @@ def batch(numX, numY, numT, s0, strike, t, alpha, nu, beta, dtype=np.float64,
     out = np.zeros(n, dtype=real)
     if n == 0:
         return out
-    rc = getattr(lib(), "lvc_batch" + suf)(n, *[_p(a) for a in ints],
-                                           *[_p(a) for a in reals], int(thomas),
-                                           int(nthreads), _p(out))
-    if rc != 0:
+    with _Grid32(grid32):
+        rc = getattr(lib(), "lvc_batch" + suf)(n, *[_p(a) for a in ints],
+                                               *[_p(a) for a in reals], int(thomas),
+                                               int(nthreads), _p(out))
+    if rc != 0 and not (allow_nan and rc == 3):
         raise ValueError(f"oracle batch failed with code {rc}")
     return out
 
 
-def init_grid(s0, alpha, nu, t, numX, numY, numT, dtype=np.float64):
+def init_grid(s0, alpha, nu, t, numX, numY, numT, dtype=np.float64, grid32=False):
     """LocVolCalib.fut:12-23 -> (xindex, yindex, x, y, timeline)."""
     suf, real = _suffix(dtype)
     x = np.zeros(numX, real)
@@ -170,8 +200,9 @@ def init_grid(s0, alpha, nu, t, numX, numY, numT, dtype=np.float64):
     tl = np.zeros(numT, real)
     xi = np.zeros(1, np.int32)
     yi = np.zeros(1, np.int32)
-    getattr(lib(), "lvc_init_grid" + suf)(s0, alpha, nu, t, numX, numY, numT,
-                                          _p(xi), _p(yi), _p(x), _p(y), _p(tl))
+    with _Grid32(grid32):
+        getattr(lib(), "lvc_init_grid" + suf)(s0, alpha, nu, t, numX, numY, numT,
+                                              _p(xi), _p(yi), _p(x), _p(y), _p(tl))
     return int(xi[0]), int(yi[0]), x, y, tl
 
 

@@ -16,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import load_dataset
+from conftest import GRID32_EDGE, GRID32_NX, grid32_tolerance, load_dataset
 from oracle import oracle
 import locvolcalib as lvc
 
@@ -36,14 +36,16 @@ def say(msg):
 
 
 def oracle_strikes(nx, ny, nt, idx, dtype=np.float64, s0=0.03, t=5.0, alpha=0.2, nu=0.6,
-                   beta=0.5, strike_dtype=None):
-    """FP64 oracle prices for strikes 0.001*i, i in idx, with the strike
-    formed in `strike_dtype` (float32 for the reference's f32 entry, :247)."""
+                   beta=0.5, strike_dtype=None, grid32=False, thomas=True):
+    """Oracle prices for strikes 0.001*i, i in idx, with the strike formed in
+    `strike_dtype` (float32 for the reference's f32 entry, :247); `grid32`:
+    initGrid's scalars in f32 (the form of the f32 entry)."""
     n = len(idx)
     sreal = np.dtype(strike_dtype or dtype).type
     K = (sreal(0.001) * np.asarray(idx).astype(sreal)).astype(np.float64)
     return oracle.batch([nx] * n, [ny] * n, [nt] * n, [s0] * n, K, [t] * n, [alpha] * n,
-                        [nu] * n, [beta] * n, dtype=dtype, thomas=True, nthreads=NTH)
+                        [nu] * n, [beta] * n, dtype=dtype, thomas=thomas, nthreads=NTH,
+                        grid32=grid32)
 
 
 def rel_err(got, ref):
@@ -117,6 +119,91 @@ def test_fp32_plain_vs_fp64_oracle():
     p = {k: float(np.float32(v)) for k, v in CFG.items()}
     ref = oracle_strikes(256, 256, 64, idx, strike_dtype=np.float32, **p)
     assert rel_err(got[idx], ref) < F32_REL
+
+
+# ---- the f32 entry's initGrid index (LocVolCalib.fut:15-20) ---------------
+# futhark_entry_main takes f32 parameters and forms stdX, dx and
+# myXindex = i32.f32(s0/dx) in f32, as the reference does; the grid arrays
+# are FP64 from those scalars.  Checked here: the index the device's gather
+# read (debugging report) against the f32 restatement's, bit for bit; the
+# prices against the FP64 oracle of the same form (grid32) at the precision
+# bars, and against the f32 restatement at its own grid-rounding error.
+
+def _f32(*v):
+    return [float(np.float32(x)) for x in v]
+
+
+def _f32_entry_checks(ctx, outer, nx, ny, nt, p, bar):
+    s0, t, a, nu, b = p
+    got = ctx.main(outer, nx, ny, nt, s0, t, a, nu, b).astype(np.float64)
+    xi32 = oracle.init_grid(*map(np.float32, (s0, a, nu, t)), nx, ny, nt,
+                            dtype=np.float32)[0]
+    assert ctx.price_index() == (ny // 2, xi32)
+    idx = np.arange(outer)
+    ref = oracle_strikes(nx, ny, nt, idx, s0=s0, t=t, alpha=a, nu=nu, beta=b,
+                         strike_dtype=np.float32, grid32=True)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < bar, err
+    ref32 = oracle.main(outer, nx, ny, nt, *map(np.float32, p), dtype=np.float32,
+                        nthreads=NTH).astype(np.float64)
+    assert np.max(np.abs(got - ref32)) < grid32_tolerance(nx)
+    return err
+
+
+@pytest.mark.parametrize("nx", GRID32_NX)
+@pytest.mark.parametrize("precision", [lvc.PRECISION_FP32_REFINE, lvc.PRECISION_FP32])
+def test_f32_entry_index_at_edge(nx, precision):
+    """t = 4, alpha = 0.2: s0/dx = numX/8 is integral in f32 (the FP64 index
+    would be numX/8 - 1, a ~2% price error)."""
+    say(f"case: f32 entry numX={nx} t=4 alpha=0.2 precision={precision}")
+    bar = F32R_REL if precision == lvc.PRECISION_FP32_REFINE else F32_REL
+    with lvc.LocVolCalib(precision=precision, debugging=True) as ctx:
+        for t, a in GRID32_EDGE[:3]:
+            _f32_entry_checks(ctx, 8, nx, 32, 12, _f32(0.03, t, a, 0.6, 0.5), bar)
+
+
+def test_f32_entry_index_random():
+    """64 seeded parameter sets across numX 32..512 (FP32 + refinement)."""
+    say("case: f32 entry, 64 random parameter sets")
+    rng = np.random.default_rng(2024)
+    worst = 0.0
+    with lvc.LocVolCalib(debugging=True) as ctx:
+        for _ in range(64):
+            nx = int(rng.choice(GRID32_NX))
+            ny = int(rng.choice([32, 64]))
+            nt = int(rng.integers(6, 20))
+            p = _f32(rng.uniform(0.02, 0.05), rng.uniform(1, 10), rng.uniform(0.1, 0.3),
+                     rng.uniform(0.4, 0.8), rng.uniform(0.3, 0.7))
+            xi = oracle.init_grid(*map(np.float32, (p[0], p[2], p[3], p[1])), nx, ny, nt,
+                                  dtype=np.float32)[0]
+            if not 0 <= xi < nx:
+                continue
+            worst = max(worst, _f32_entry_checks(ctx, 4, nx, ny, nt, p, F32R_REL))
+    say(f"   worst rel err vs the FP64 grid32 oracle {worst:.3e}")
+
+
+def test_main_price_index_out_of_bounds(gpu_ctx):
+    """20*alpha*sqrt(t) = 1: s0/dx = numX, so myXindex = numX lies outside the
+    grid; the reference's bounds-checked gather (:236) fails the call, and so
+    do both main entries; the batch entry gives NaN for that problem."""
+    say("case: price index out of bounds")
+    p = (0.03, 0.25, 0.1, 0.6, 0.5)
+    with pytest.raises(lvc.LocVolCalibError, match="out of bounds"):
+        gpu_ctx.main(4, 32, 32, 8, *p)
+    with pytest.raises(lvc.LocVolCalibError, match="out of bounds"):
+        gpu_ctx.main_f64(4, 32, 32, 8, *p)
+    assert gpu_ctx.main(0, 32, 32, 8, *p).shape == (0,)   # no option, no index
+    n = 3
+    pr = dict(numX=np.full(n, 32, np.int32), numY=np.full(n, 32, np.int32),
+              numT=np.full(n, 8, np.int32), s0=np.full(n, 0.03), strike=np.full(n, 0.01),
+              t=np.array([0.25, 5.0, 0.25]), alpha=np.array([0.1, 0.2, 0.1]),
+              nu=np.full(n, 0.6), beta=np.full(n, 0.5))
+    got = gpu_ctx.main_batch(**pr, precision=lvc.PRECISION_FP64)
+    ref = oracle.batch(pr["numX"], pr["numY"], pr["numT"], pr["s0"], pr["strike"], pr["t"],
+                       pr["alpha"], pr["nu"], pr["beta"], dtype=np.float64, thomas=True,
+                       allow_nan=True)
+    assert np.isnan(got[0]) and np.isnan(got[2]) and np.isnan(ref[0])
+    assert abs(got[1] - ref[1]) / abs(ref[1]) < F64_REL
 
 
 # ---- size-independent properties at full size ----------------------------

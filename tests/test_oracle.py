@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_dataset
+from conftest import GOLDEN, GRID32_EDGE, GRID32_NX, grid32_tolerance, load_dataset
 from oracle import oracle
 from locvolcalib import futhark_values as fv
 
@@ -203,6 +203,73 @@ def test_update_params():
 
 
 # ---- tridagPar / tridagSeq --------------------------------------------------
+
+# ---- initGrid's scalars in f32 (the reference's f32 entry) ------------------
+# GRID32_EDGE, GRID32_NX and grid32_tolerance: see conftest.py.
+
+def _f32(*v):
+    return [float(np.float32(x)) for x in v]
+
+
+def test_grid32_index_is_the_f32_index():
+    """grid32 mode's myXindex equals the f32 restatement's on the edge
+    parameters (where the FP64 index is one less) and on random ones, and
+    x[myXindex] == s0 exactly (:21)."""
+    rng = np.random.default_rng(11)
+    params = [(0.03, t, a) for t, a in GRID32_EDGE]
+    params += [tuple(rng.uniform(lo, hi) for lo, hi in ((0.02, 0.05), (1, 10), (0.1, 0.3)))
+               for _ in range(40)]
+    nedge_diff = 0
+    for s0, t, a in params:
+        s0, t, a, nu = _f32(s0, t, a, 0.6)
+        for nx in GRID32_NX:
+            xi32 = oracle.init_grid(np.float32(s0), np.float32(a), np.float32(nu), np.float32(t),
+                                    nx, 16, 8, dtype=np.float32)[0]
+            xi, _, x, _, _ = oracle.init_grid(s0, a, nu, t, nx, 16, 8, dtype=np.float64,
+                                              grid32=True)
+            xi64 = oracle.init_grid(s0, a, nu, t, nx, 16, 8, dtype=np.float64)[0]
+            assert xi == xi32
+            nedge_diff += xi64 != xi32
+            if 0 <= xi < nx:
+                assert x[xi] == s0
+    assert nedge_diff >= len(GRID32_EDGE) * len(GRID32_NX)
+
+
+def test_grid32_closed_form_at_edge():
+    """At the edge parameters x[0] = 0 exactly (s0/dx integral in f32), so a
+    strike K = 0 lies on the grid and the price is s0 - K = s0 in exact
+    arithmetic (payoff linear, every operator row annihilates it)."""
+    for t, a in GRID32_EDGE[:3]:
+        p = _f32(0.03, t, a, 0.6, 0.5)
+        got = oracle.main(2, 32, 32, 8, p[0], p[1], p[2], p[3], p[4], dtype=np.float64,
+                          thomas=True, grid32=True)
+        assert abs(got[0] - p[0]) < 1e-12
+
+
+@pytest.mark.parametrize("nx", GRID32_NX)
+def test_grid32_vs_f32_restatement(nx):
+    """The FP64-grid form of the reference's f32 program agrees with the f32
+    restatement at the f32 grid's own rounding error, on the edge parameters
+    and random ones; the pure FP64 restatement (index off by one at the edge)
+    does not."""
+    rng = np.random.default_rng(nx)
+    sets = [_f32(0.03, 4.0, 0.2, 0.6, 0.5)]
+    for _ in range(6):
+        sets.append(_f32(rng.uniform(0.02, 0.05), rng.uniform(1, 10), rng.uniform(0.1, 0.3),
+                         rng.uniform(0.4, 0.8), rng.uniform(0.3, 0.7)))
+    tol = grid32_tolerance(nx)
+    nth = os.cpu_count() or 1
+    for i, (s0, t, a, nu, b) in enumerate(sets):
+        p32 = oracle.main(6, nx, 32, 12, *map(np.float32, (s0, t, a, nu, b)),
+                          dtype=np.float32, nthreads=nth)
+        pg = oracle.main(6, nx, 32, 12, s0, t, a, nu, b, dtype=np.float64, thomas=True,
+                         nthreads=nth, grid32=True)
+        assert np.max(np.abs(p32 - pg)) < tol
+        if i == 0:
+            p64 = oracle.main(6, nx, 32, 12, s0, t, a, nu, b, dtype=np.float64, thomas=True,
+                              nthreads=nth)
+            assert np.max(np.abs(p32 - p64)) > 1.5 * tol
+
 
 def _dd_system(rng, n, dtype=np.float64):
     a = -rng.uniform(0.1, 1.0, n)
