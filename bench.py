@@ -13,10 +13,20 @@ Reported:
               ratio; p50/p99 per-call latency of both;
   roofline    the dominant kernel (the x sweep of the FP32+refine form):
               algorithmic bytes per launch (read R + write Y, 2*N*4 bytes per
-              option, 128 options) / its mean duration from HIP events
-              recorded around every launch on the library's stream during the
-              timed calls, against 8 TB/s;
-  accuracy    max relative price difference FP32+refine vs FP64;
+              option, 128 options) / its mean launch duration, against 8 TB/s.
+              The duration: the event-free timed calls' time per call times
+              the kernel's share of the GPU time per call, the share from HIP
+              events recorded around every launch on the library's stream in
+              a second pass of the same calls (a call is its kernels back to
+              back, with no gaps between them, so the event-free call time is
+              their sum; the events themselves lengthen each launch);
+              `valu` beside it from the committed rocprofv3 counters;
+  accuracy    per workload, max abs / relative price error against the FP64
+              oracle (the C restatement, Thomas form);
+  workloads   main(16, 32, 256, 256) (the CPU-sized plain method),
+              main(256, 256, 256, 128) (the precision form), the batch of 512
+              uneven problems, the served Poisson stream at six rates and at
+              saturation;
   cpu_baseline  the oracle (C restatement of LocVolCalib.fut, scan-form
               tridiagonal solve as the reference) on the host cores, OpenMP
               over options, on a time-bounded sample of the same workload.
@@ -34,6 +44,11 @@ sys.path.insert(0, ROOT)
 OUTER, NX, NY, NT = 128, 256, 256, 64
 PARAMS = (0.03, 5.0, 0.2, 0.6, 0.5)
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def time_calls(ctx, raw, free, steps, warmup):
@@ -64,6 +79,85 @@ def profiled_traffic(kernel):
             return json.load(f)["hbm_bytes_per_launch"].get(kernel)
     except (OSError, ValueError, KeyError):
         return None
+
+
+def profiled_valu(kernel):
+    """VALU issue fraction of `kernel` from the committed rocprofv3 counters
+    (SQ_INSTS_VALU over the SIMDs' issue slots in its measured duration)."""
+    path = os.path.join(ROOT, "profiles",
+                        "lvc_sweeps_main_128x256x256x64_fp32refine.traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["valu"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def accuracy(got, ref):
+    """Max abs and relative (per price, and to the largest price) error of
+    `got` against the oracle's `ref`."""
+    import numpy as np
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    fin = np.isfinite(ref)
+    d = np.abs(got[fin] - ref[fin])
+    nz = ref[fin] != 0
+    return {"max_abs": float(d.max()) if d.size else 0.0,
+            "max_rel": float((d[nz] / np.abs(ref[fin][nz])).max()) if nz.any() else 0.0,
+            "max_rel_to_max_price": float(d.max() / np.abs(ref[fin]).max()) if d.size else 0.0,
+            "prices": int(fin.sum())}
+
+
+def oracle_main(outer, nx, ny, nt, params, f32_entry, nthreads=16):
+    """The FP64 oracle's prices of main's options: with `f32_entry` in the
+    f32 entry's form (f32 parameters and strikes 0.001f*i, initGrid's scalars
+    in f32), else the FP64 entry's."""
+    import numpy as np
+    from oracle import oracle
+    idx = np.arange(outer)
+    if f32_entry:
+        p = [float(np.float32(v)) for v in params]
+        K = (np.float32(0.001) * idx.astype(np.float32)).astype(np.float64)
+    else:
+        p = list(params)
+        K = 0.001 * idx.astype(np.float64)
+    return oracle.batch([nx] * outer, [ny] * outer, [nt] * outer, [p[0]] * outer, K,
+                        [p[1]] * outer, [p[2]] * outer, [p[3]] * outer, [p[4]] * outer,
+                        dtype=np.float64, thomas=True, nthreads=nthreads, grid32=f32_entry)
+
+
+def bench_main_workload(lvc, outer, nx, ny, nt, params=PARAMS, reps=10, check=True):
+    """main and main_f64 on one workload: ms per call, options/s, and their
+    errors against the FP64 oracle."""
+    import numpy as np
+    out = {"workload": f"main({outer}, {nx}, {ny}, {nt})"}
+    res = {}
+    with lvc.LocVolCalib(device=0) as ctx:
+        for key, raw, free, get in (("fp32refine", ctx.main_raw, ctx.free_f32, ctx.main),
+                                    ("fp64", ctx.main_f64_raw, ctx.free_f64, ctx.main_f64)):
+            for _ in range(2):
+                free(raw(outer, nx, ny, nt, *params))
+            ctx.sync()
+            lat = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                h = raw(outer, nx, ny, nt, *params)
+                ctx.sync()
+                lat.append(time.perf_counter() - t0)
+                free(h)
+            res[key] = get(outer, nx, ny, nt, *params)
+            out[key] = {"ms_per_call": round(1e3 * float(np.median(lat)), 4),
+                        "options_per_s": round(outer / float(np.median(lat)), 1)}
+    out["ratio_fp32refine_vs_fp64"] = round(out["fp32refine"]["options_per_s"] /
+                                            out["fp64"]["options_per_s"], 3)
+    log(f"{out['workload']}: {out['fp32refine']['ms_per_call']} / {out['fp64']['ms_per_call']} ms")
+    if check:
+        log(f"{out['workload']}: FP64 oracle")
+        out["fp32refine"]["accuracy_vs_fp64_oracle"] = accuracy(
+            res["fp32refine"], oracle_main(outer, nx, ny, nt, params, True))
+        out["fp64"]["accuracy_vs_fp64_oracle"] = accuracy(
+            res["fp64"], oracle_main(outer, nx, ny, nt, params, False))
+    return out
 
 
 def pct(v, q):
@@ -130,19 +224,30 @@ def uneven_problems(n, seed, sizes=(32, 64, 128, 256), nt=(32, 256)):
                 nu=rng.uniform(0.4, 0.8, n), beta=rng.uniform(0.3, 0.7, n))
 
 
-def bench_batched(lvc, reps=3, n=512, seed=42):
-    """512 uneven problems in one batch call (FP32 + refinement)."""
+def bench_batched(lvc, reps=3, n=512, seed=42, check=True):
+    """512 uneven problems in one batch call (FP32 + refinement), and its
+    errors against the FP64 oracle."""
+    import numpy as np
     pr = uneven_problems(n, seed)
     with lvc.LocVolCalib(device=0) as ctx:
-        ctx.main_batch(**pr)                       # warm-up (and tuning)
+        got = ctx.main_batch(**pr)                 # warm-up (and tuning)
         t0 = time.perf_counter()
         for _ in range(reps):
             ctx.main_batch(**pr)
         dt = (time.perf_counter() - t0) / reps
     elems = float((pr["numX"].astype(float) * pr["numY"] * (pr["numT"] - 1)).sum())
-    return {"problems": n, "seed": seed, "ms_per_batch": round(dt * 1e3, 3),
-            "problems_per_s": round(n / dt, 2),
-            "grid_point_steps_per_s": round(elems / dt, 1)}
+    out = {"problems": n, "seed": seed, "ms_per_batch": round(dt * 1e3, 3),
+           "problems_per_s": round(n / dt, 2),
+           "grid_point_steps_per_s": round(elems / dt, 1)}
+    log(f"batched: {out['ms_per_batch']} ms per batch")
+    if check:
+        log("batched: FP64 oracle")
+        from oracle import oracle
+        keys = ("numX", "numY", "numT", "s0", "strike", "t", "alpha", "nu", "beta")
+        ref = oracle.batch(*[pr[k] for k in keys], dtype=np.float64, thomas=True,
+                           nthreads=16, allow_nan=True)
+        out["accuracy_vs_fp64_oracle"] = accuracy(got, ref)
+    return out
 
 
 def bench_ocean(lvc, n=57600, m=115, reps=20, seed=3):
@@ -222,46 +327,82 @@ def bench_option_pricing(reps=10, cpu=True):
 
 
 def bench_serving(lvc, rate, duration_s=1.5, seed=7, max_slots=1024, budget=64 << 30):
-    """Poisson arrivals at `rate` req/s of single problems (sizes and
-    parameters as the batched set), continuous batching, FP32 + refinement.
-    Latency = completion - arrival (host clock)."""
+    """Poisson arrivals at `rate` req/s (None: every request queued at time 0,
+    the saturated server) of single problems (sizes and parameters as the
+    batched set), continuous batching, FP32 + refinement.  Latency =
+    completion - arrival (host clock); completions/s over the whole run."""
+    import heapq
     import numpy as np
     rng = np.random.default_rng(seed)
-    n = max(1, int(rate * duration_s))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    n = max(1, int((rate or 20000) * duration_s))
+    arrivals = (np.cumsum(rng.exponential(1.0 / rate, n)) if rate
+                else np.zeros(n))
     pr = uneven_problems(n, seed + 1)
     keys = ("numX", "numY", "numT", "s0", "strike", "t", "alpha", "nu", "beta")
     reqs = [tuple(int(pr[k][i]) if k.startswith("num") else float(pr[k][i]) for k in keys)
             for i in range(n)]
     with lvc.LocVolCalib(device=0) as ctx:
         srv = lvc.Server(ctx, max_slots=max_slots, mem_budget=budget)
-        warm = srv.submit(*reqs[0])
+        srv.submit(*reqs[0])
         srv.drain()
         ids, done_at = [None] * n, [None] * n
-        pending = set()
-        nxt = 0
+        # requests are admitted in submission order: those before `adm` are
+        # admitted; `due` holds (earliest tick it can complete, index) for them
+        due = []
+        nxt, adm, tick, left = 0, 0, 0, n
+        base = srv.stats()["completed"]
         t0 = time.perf_counter()
-        while nxt < n or pending:
+        t_log = t0
+        while left:
             now = time.perf_counter() - t0
+            if now + t0 - t_log > 10.0:
+                t_log = now + t0
+                log(f"serving at {rate or 'saturation'}: {n - left} of {n} done, tick {tick}")
             while nxt < n and arrivals[nxt] <= now:
                 ids[nxt] = srv.submit(*reqs[nxt])
-                pending.add(nxt)
                 nxt += 1
-            if not pending:
+            if nxt == n - left:                # nothing outstanding: wait
                 time.sleep(max(0.0, min(arrivals[nxt] - now, 0.001)))
                 continue
-            if srv.step() > 0:
+            done = srv.step()
+            st = srv.stats()
+            admitted = st["completed"] - base + st["resident"]
+            while adm < admitted:          # admitted by this tick: n_t - 1 ticks from it
+                heapq.heappush(due, (tick + reqs[adm][2] - 2, adm))
+                adm += 1
+            if done > 0:
                 now = time.perf_counter() - t0
-                for i in list(pending):
+                while due and due[0][0] <= tick:
+                    _, i = heapq.heappop(due)
                     if srv.poll(ids[i]) is not None:
                         done_at[i] = now
-                        pending.discard(i)
+                        left -= 1
+                    else:
+                        heapq.heappush(due, (tick + 1, i))
+            tick += 1
         total = time.perf_counter() - t0
         srv.free()
     lat = (np.array(done_at) - arrivals) * 1e3
-    return {"rate_req_per_s": rate, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+    log(f"serving at {rate or 'saturation'}: {n} requests, {n / total:.0f} completed/s")
+    return {"rate_req_per_s": rate if rate else "saturated (all queued at t=0)",
+            "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "options_per_s": round(n / total, 2), "wall_s": round(total, 3)}
+            "options_per_s": round(n / total, 2), "wall_s": round(total, 3),
+            "ticks": tick}
+
+
+def bench_serving_sweep(lvc, rates):
+    """The given rates, then doubling from the last one until completions/s
+    falls below 95% of the arrival rate, then the saturated server."""
+    runs = [bench_serving(lvc, float(r)) for r in rates]
+    r = float(rates[-1])
+    while runs[-1]["options_per_s"] >= 0.95 * r and r < 64e3:
+        r *= 2
+        runs.append(bench_serving(lvc, r))
+    sat = bench_serving(lvc, None)
+    keeps = [x["rate_req_per_s"] for x in runs if x["options_per_s"] >= 0.95 * x["rate_req_per_s"]]
+    return {"runs": runs, "saturated": sat,
+            "highest_rate_kept_up": max(keeps) if keeps else None}
 
 
 def main():
@@ -272,7 +413,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched and serving measurements")
-    ap.add_argument("--serve-rates", default="250,1000,5000",
+    ap.add_argument("--serve-rates", default="100,250,500,1000,2500,5000",
                     help="Poisson arrival rates (req/s) of the serving runs")
     ap.add_argument("--only-fp32refine", action="store_true",
                     help="run only the FP32+refinement calls (for profiling)")
@@ -327,7 +468,13 @@ def main():
     ys = rep32.get("lvc_ysweep", {"count": 0, "total_us": 0.0})
     dom, other = (xs, ys) if xs["total_us"] >= ys["total_us"] else (ys, xs)
     dom_name = "lvc_xsweep" if dom is xs else "lvc_ysweep"
-    avg_us = dom["total_us"] / max(dom["count"], 1)
+    avg_us_events = dom["total_us"] / max(dom["count"], 1)
+    # the kernel's share of the event-timed GPU time, applied to the
+    # event-free call time (the calls' kernels run back to back)
+    gpu_us = sum(v["total_us"] for v in rep32.values())
+    share = dom["total_us"] / gpu_us if gpu_us > 0 else 0.0
+    launches_per_call = dom["count"] / max(args.steps, 1)
+    avg_us = (1e6 * total32 / args.steps) * share / max(launches_per_call, 1e-9)
     bytes_launch = OUTER * 2 * NX * NY * 4
     achieved = bytes_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     rel = float(np.max(np.abs(p32 - p64) / np.abs(p64)))
@@ -357,21 +504,38 @@ def main():
                  "p50_ms": round(pct(lat64, 50), 4), "p99_ms": round(pct(lat64, 99), 4)},
         "ratio_fp32refine_vs_fp64": round(v32 / v64, 3),
         "max_rel_err_fp32refine_vs_fp64": rel,
-        "kernels_us_per_launch": {k: round(v["total_us"] / max(v["count"], 1), 3)
-                                  for k, v in rep32.items()},
+        "kernels_us_per_launch_with_events": {k: round(v["total_us"] / max(v["count"], 1), 3)
+                                              for k, v in rep32.items()},
         "ms_per_step_with_events": round(1e3 * total32p / args.steps, 4),
         "roofline": {"bound": "hbm", "kernel": dom_name,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(dom_name),
-                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3)},
+                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3),
+                     "avg_launch_us_how": "event-free time per call x the kernel's share of "
+                                          "the event-timed GPU time / its launches per call",
+                     "avg_launch_us_with_events": round(avg_us_events, 3),
+                     "valu": profiled_valu(dom_name)},
     }
+    log(f"headline: {out['ms_per_step']} ms per call (FP32+refine), "
+        f"{out['fp64']['ms_per_step']} ms (FP64)")
+    check = not args.no_cpu_baseline
+    if check:
+        log("headline: FP64 oracle")
+        out["accuracy_vs_fp64_oracle"] = {
+            "fp32refine": accuracy(p32, oracle_main(OUTER, NX, NY, NT, PARAMS, True)),
+            "fp64": accuracy(p64, oracle_main(OUTER, NX, NY, NT, PARAMS, False))}
     if not args.no_forms:
+        out["plain_small"] = bench_main_workload(lvc, 16, 32, 256, 256, check=check)
+        out["precision_form"] = bench_main_workload(lvc, 256, 256, 256, 128, check=check)
+        out["batched"] = bench_batched(lvc, check=check)
+        out["serving"] = bench_serving_sweep(lvc, args.serve_rates.split(","))
+        log("ocean-sim tridiagonal entries")
         out["ocean_tridiag"] = bench_ocean(lvc)
-        out["option_pricing"] = bench_option_pricing(cpu=not args.no_cpu_baseline)
-        out["batched"] = bench_batched(lvc)
-        out["serving"] = [bench_serving(lvc, float(r)) for r in args.serve_rates.split(",")]
+        log("OptionPricing")
+        out["option_pricing"] = bench_option_pricing(cpu=check)
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 

@@ -234,6 +234,15 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, T v, int 
   }
 }
 
+// A wave-uniform 64-bit value (a descriptor field) in SGPRs: the compiler
+// cannot always prove such a load uniform, and a buffer resource built from
+// a value it holds in vector registers costs a uniformity loop per access.
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)v);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(((uint64_t)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // Coefficient tables of the geometry buffer in the sweep's real type.
 template <typename T>
 struct Taps {
@@ -969,19 +978,20 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   // with the refinement straight from global memory (scalar loads into
   // SGPRs, operands of the vector arithmetic directly, which keeps VGPRs for
   // the refinement), without it from a copy in LDS in the same layout
-  // (measured faster there).  The copy takes one row per thread (P * M <=
+  // (measured faster there; with the refinement the two are equal).  The copy takes one row per thread (P * M <=
   // 64 * P, no more than the threads still running) and loads from valid
   // addresses only (clamped indexes; threads past the rows write row 0
   // again), so it stays straight-line and its loads are issued before the
   // column's: the LDS writes and the barrier wait for them alone (loads
   // complete in order).
   const T *__restrict__ f = yfac + (int64_t)pr.fac_src * LVC_YFAC_LEN;
-  __shared__ T ftab[REF ? 1 : LVC_YFAC_LEN];
+  constexpr bool kFacLds = !REF;
+  __shared__ T ftab[kFacLds ? LVC_YFAC_LEN : 1];
   const int fr = threadIdx.x;
   const int fi = fr < P * M ? fr : 0, si = fr < P ? fr : 0;
   constexpr int kRowFields = kFw + 1, kSegFields = kSrC + 1;  // the fields the solve reads
   T fv[kRowFields] = {}, sv[kSegFields] = {};
-  if (!REF) {
+  if (kFacLds) {
 #pragma unroll
     for (int q = 0; q < kRowFields; ++q) fv[q] = f[q * LVC_COL_ROWS + fi];
 #pragma unroll
@@ -1007,12 +1017,13 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   // (finite) values stay in their own column, which is never stored.
   const int lane_off = (active ? col : tile.first) * (int)sizeof(T);
   const int row_bytes = nx * (int)sizeof(T);
-  const __amdgpu_buffer_rsrc_t in = grid_rsrc(Y + pr.state_off, nx, ny);
+  const int64_t state_off = wave_uniform(pr.state_off);
+  const __amdgpu_buffer_rsrc_t in = grid_rsrc(Y + state_off, nx, ny);
   T d[M];
 #pragma unroll
   for (int k = 0; k < M; ++k)
     d[k] = buffer_load<T>(in, lane_off, min(r0 + k, ny - 1) * row_bytes);
-  if (!REF) {
+  if (kFacLds) {
 #pragma unroll
     for (int q = 0; q < kRowFields; ++q) ftab[q * LVC_COL_ROWS + fi] = fv[q];
 #pragma unroll
@@ -1022,7 +1033,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
     if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
   }
-  const T *F = REF ? f : ftab;
+  const T *F = kFacLds ? ftab : f;
   if (r0 + M > ny) {  // wave-uniform
 #pragma unroll
     for (int k = 0; k < M; ++k)
@@ -1055,8 +1066,8 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] += d[k];
   }
+  const __amdgpu_buffer_rsrc_t out = grid_rsrc(R + state_off, nx, ny);
   if (active) {
-    const __amdgpu_buffer_rsrc_t out = grid_rsrc(R + pr.state_off, nx, ny);
     if (r0 + M <= ny) {  // wave-uniform
 #pragma unroll
       for (int k = 0; k < M; ++k) buffer_store<T>(out, x[k], lane_off, (r0 + k) * row_bytes);

@@ -3,14 +3,20 @@
  * (each one LocVolCalib.fut:211-236 `value` with its own sizes and
  * parameters).  No counterpart in the reference.
  *
- * The server owns `max_slots` fixed device slots.  A slot holds one resident
- * request: its descriptor, its geometry and its two state grids (R, Y), all
- * sized for the largest accepted grid, so admission and completion never
- * move another request's data.  One engine tick (lvc_server_step):
+ * The server owns `max_slots` device slots and one arena for the resident
+ * grids.  A slot holds a resident request's descriptor, geometry and y factor
+ * table (fixed size, for the largest accepted grid); its two state grids
+ * (R, Y) take an extent of the arena of exactly their own size, found first
+ * fit, so admission and completion never move another request's data.  All
+ * of it is allocated when the server is made, and the memory budget bounds
+ * the total: the slot tables first, the arena from what remains (at most
+ * what max_slots grids of 512 x 512 would take).  One engine tick
+ * (lvc_server_step):
  *   1. admits queued requests, in arrival order, while a slot is free and the
- *      resident bytes (the sum of the admitted requests' own grid sizes) stay
- *      within the memory budget; each admission uploads its descriptor and
- *      runs the setup kernel for that slot;
+ *      arena has an extent for the request's grids (in-order: the first
+ *      request that does not fit waits for memory, and the ones behind it
+ *      with it); each admission uploads its descriptor and runs the setup
+ *      kernel for that slot;
  *   2. launches the x sweep and the y sweep once over every resident request
  *      (each advances one time step);
  *   3. gathers the price of every request that has just done its last step,
@@ -27,6 +33,14 @@
 #include "lvc_runtime.h"
 
 #define SLOT_MAX_N 512 /* largest numX / numY a server slot holds */
+/* tiles of one request at most: x tiles ny / (rows per tile >= 4), y tiles
+ * nx / (columns per tile >= 4) */
+#define SLOT_MAX_TILES (2 * SLOT_MAX_N / 4)
+
+/* a free extent of the grid arena, in elements */
+typedef struct {
+  int64_t off, len;
+} extent;
 
 typedef struct {
   int64_t id;
@@ -43,7 +57,12 @@ struct lvc_server {
   int max_slots, precision, seg_x, seg_y;
   int64_t budget, resident_bytes;
   size_t real;
-  int64_t slot_elems, slot_geo;
+  int64_t slot_geo;
+  /* grid arena: R and Y at the same element offsets of d_R and d_Y */
+  int64_t arena_elems, device_bytes;
+  extent *free_ext; /* sorted by offset, disjoint, not adjacent */
+  int nfree;
+  int64_t *slot_len; /* arena elements held by each slot */
   /* slots */
   lvc_problem *h_slots;
   int64_t *slot_id;  /* request id, -1 when free */
@@ -68,9 +87,63 @@ struct lvc_server {
   int64_t completed;
 };
 
-static size_t request_bytes(const struct lvc_server *s, const struct lvc_request *r) {
-  return 2 * (size_t)r->numX * (size_t)r->numY * s->real +
-         (size_t)LVC_GEO_LEN(r->numX, r->numY) * sizeof(double);
+/* arena elements of a request's grids (64-element aligned extents) */
+static int64_t request_elems(int nx, int ny) {
+  return ((int64_t)nx * ny + 63) & ~(int64_t)63;
+}
+
+/* bytes of a request's grids R and Y in the arena */
+static int64_t request_bytes(const struct lvc_server *s, int nx, int ny) {
+  return 2 * request_elems(nx, ny) * (int64_t)s->real;
+}
+
+/* device bytes of the per-slot tables: descriptor, geometry, y factor
+ * table, output, factor list and tiles */
+static int64_t slot_bytes(const struct lvc_server *s, int col) {
+  return (int64_t)sizeof(lvc_problem) + s->slot_geo * (int64_t)sizeof(double) +
+         (col ? LVC_YFAC_LEN * (int64_t)s->real + (int64_t)sizeof(int32_t) : 0) +
+         (int64_t)sizeof(double) + SLOT_MAX_TILES * (int64_t)sizeof(lvc_tile);
+}
+
+/* First-fit extent of `len` elements; -1 when none is free. */
+static int64_t arena_take(struct lvc_server *s, int64_t len) {
+  for (int i = 0; i < s->nfree; i++) {
+    if (s->free_ext[i].len < len)
+      continue;
+    const int64_t off = s->free_ext[i].off;
+    s->free_ext[i].off += len;
+    s->free_ext[i].len -= len;
+    if (s->free_ext[i].len == 0) {
+      memmove(s->free_ext + i, s->free_ext + i + 1, (size_t)(s->nfree - i - 1) * sizeof(extent));
+      s->nfree--;
+    }
+    return off;
+  }
+  return -1;
+}
+
+/* Returns an extent, merging it with free neighbours. */
+static void arena_give(struct lvc_server *s, int64_t off, int64_t len) {
+  int i = 0;
+  while (i < s->nfree && s->free_ext[i].off < off)
+    i++;
+  const int prev = i > 0 && s->free_ext[i - 1].off + s->free_ext[i - 1].len == off;
+  const int next = i < s->nfree && off + len == s->free_ext[i].off;
+  if (prev && next) {
+    s->free_ext[i - 1].len += len + s->free_ext[i].len;
+    memmove(s->free_ext + i, s->free_ext + i + 1, (size_t)(s->nfree - i - 1) * sizeof(extent));
+    s->nfree--;
+  } else if (prev) {
+    s->free_ext[i - 1].len += len;
+  } else if (next) {
+    s->free_ext[i].off = off;
+    s->free_ext[i].len += len;
+  } else {
+    memmove(s->free_ext + i + 1, s->free_ext + i, (size_t)(s->nfree - i) * sizeof(extent));
+    s->free_ext[i].off = off;
+    s->free_ext[i].len = len;
+    s->nfree++;
+  }
 }
 
 struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots,
@@ -90,8 +163,7 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
   s->precision = precision;
   s->budget = mem_budget_bytes;
   s->real = precision == LVC_FP64 ? sizeof(double) : sizeof(float);
-  s->slot_elems = (int64_t)SLOT_MAX_N * SLOT_MAX_N;
-  s->slot_geo = LVC_GEO_LEN(SLOT_MAX_N, SLOT_MAX_N);
+  s->slot_geo = (LVC_GEO_LEN(SLOT_MAX_N, SLOT_MAX_N) + 63) & ~(int64_t)63;
   pthread_mutex_lock(&ctx->lock);
   /* segment lengths fixed for the server's life (one kernel variant for the
    * whole resident set); tuned for the largest slot grid */
@@ -100,14 +172,42 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
     s->seg_x = 8;
   if (!lvc_yseg_fits(SLOT_MAX_N, s->seg_y))
     s->seg_y = lvc_seg_is_col(s->seg_y) ? 64 : 8;
+  /* the budget: the slot tables first, the grid arena from the rest, at most
+   * what max_slots grids of the largest size take */
+  const int col = lvc_seg_is_col(s->seg_y);
+  const int64_t fixed = (int64_t)max_slots * slot_bytes(s, col);
+  int64_t arena = (int64_t)max_slots * request_elems(SLOT_MAX_N, SLOT_MAX_N);
+  if (s->budget > 0) {
+    const int64_t fit = (s->budget - fixed) / (2 * (int64_t)s->real) & ~(int64_t)63;
+    if (fit < request_elems(3, 3)) {
+      pthread_mutex_unlock(&ctx->lock);
+      lvc_set_error(ctx, "lvc_server_new: a budget of %lld bytes does not cover the %lld "
+                         "bytes of the slot tables (%d slots) and a grid",
+                    (long long)s->budget, (long long)fixed, max_slots);
+      free(s);
+      return NULL;
+    }
+    if (fit < arena)
+      arena = fit;
+  }
+  s->arena_elems = arena;
+  s->device_bytes = fixed + 2 * arena * (int64_t)s->real;
   s->h_slots = calloc((size_t)max_slots, sizeof(lvc_problem));
   s->slot_id = malloc((size_t)max_slots * sizeof(int64_t));
   s->steps_left = calloc((size_t)max_slots, sizeof(int));
   s->h_out = calloc((size_t)max_slots, sizeof(double));
   s->h_fprobs = calloc((size_t)max_slots, sizeof(int32_t));
-  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_out || !s->h_fprobs;
+  s->slot_len = calloc((size_t)max_slots, sizeof(int64_t));
+  s->free_ext = malloc(((size_t)max_slots + 1) * sizeof(extent));
+  s->tiles_cap = max_slots * SLOT_MAX_TILES;
+  s->h_tiles = malloc((size_t)s->tiles_cap * sizeof(lvc_tile));
+  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_out || !s->h_fprobs ||
+           !s->slot_len || !s->free_ext || !s->h_tiles;
   if (rc)
     lvc_set_error(ctx, "host allocation failed");
+  s->free_ext[0].off = 0;
+  s->free_ext[0].len = arena;
+  s->nfree = 1;
   for (int i = 0; !rc && i < max_slots; i++) {
     s->slot_id[i] = -1;
     s->h_slots[i].k = -1;
@@ -115,9 +215,10 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
   if (!rc)
     rc = lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(lvc_problem), (void **)&s->d_slots) ||
          lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_geo * sizeof(double), (void **)&s->d_geo) ||
-         lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_elems * s->real, &s->d_R) ||
-         lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_elems * s->real, &s->d_Y) ||
+         lvc_dev_alloc(ctx, (size_t)arena * s->real, &s->d_R) ||
+         lvc_dev_alloc(ctx, (size_t)arena * s->real, &s->d_Y) ||
          lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(double), (void **)&s->d_out) ||
+         lvc_dev_alloc(ctx, (size_t)s->tiles_cap * sizeof(lvc_tile), (void **)&s->d_tiles) ||
          (lvc_seg_is_col(s->seg_y) &&
           (lvc_dev_alloc(ctx, (size_t)max_slots * LVC_YFAC_LEN * s->real, &s->d_yfac) ||
            lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(int32_t), (void **)&s->d_fprobs)));
@@ -139,8 +240,8 @@ void lvc_server_free(struct lvc_server *s) {
   (void)hipStreamSynchronize(ctx->stream);
   lvc_dev_free(ctx, s->d_slots, (size_t)s->max_slots * sizeof(lvc_problem));
   lvc_dev_free(ctx, s->d_geo, (size_t)s->max_slots * s->slot_geo * sizeof(double));
-  lvc_dev_free(ctx, s->d_R, (size_t)s->max_slots * s->slot_elems * s->real);
-  lvc_dev_free(ctx, s->d_Y, (size_t)s->max_slots * s->slot_elems * s->real);
+  lvc_dev_free(ctx, s->d_R, (size_t)s->arena_elems * s->real);
+  lvc_dev_free(ctx, s->d_Y, (size_t)s->arena_elems * s->real);
   lvc_dev_free(ctx, s->d_out, (size_t)s->max_slots * sizeof(double));
   if (s->d_yfac)
     lvc_dev_free(ctx, s->d_yfac, (size_t)s->max_slots * LVC_YFAC_LEN * s->real);
@@ -154,6 +255,8 @@ void lvc_server_free(struct lvc_server *s) {
   free(s->steps_left);
   free(s->h_out);
   free(s->h_fprobs);
+  free(s->slot_len);
+  free(s->free_ext);
   free(s->h_tiles);
   free(s->queue);
   free(s->results);
@@ -175,9 +278,11 @@ int64_t lvc_server_submit(struct lvc_server *s, const struct lvc_request *r) {
     lvc_set_error(ctx, "grid too large for the server's segment lengths");
     return -1;
   }
-  if (s->budget > 0 && (int64_t)request_bytes(s, r) > s->budget) {
-    lvc_set_error(ctx, "request needs %zu bytes, more than the budget of %lld",
-                  request_bytes(s, r), (long long)s->budget);
+  if (request_elems(r->numX, r->numY) > s->arena_elems) {
+    lvc_set_error(ctx, "request needs %lld bytes of grids, more than the %lld bytes the "
+                       "budget of %lld leaves for them",
+                  (long long)request_bytes(s, r->numX, r->numY),
+                  (long long)(2 * s->arena_elems * (int64_t)s->real), (long long)s->budget);
     return -1;
   }
   if (s->qtail == s->qcap) {
@@ -225,19 +330,10 @@ static int rebuild_tiles(struct lvc_server *s) {
     nx_t += (q->ny + rows - 1) / rows;
     ny_t += q->ytiles;
   }
-  const int need = nx_t + ny_t;
+  const int need = nx_t + ny_t; /* <= SLOT_MAX_TILES per resident request */
   if (need > s->tiles_cap) {
-    if (s->d_tiles)
-      lvc_dev_free(ctx, s->d_tiles, (size_t)s->tiles_cap * sizeof(lvc_tile));
-    s->d_tiles = NULL;
-    free(s->h_tiles);
-    s->tiles_cap = need + need / 2 + 64;
-    s->h_tiles = malloc((size_t)s->tiles_cap * sizeof(lvc_tile));
-    if (!s->h_tiles ||
-        lvc_dev_alloc(ctx, (size_t)s->tiles_cap * sizeof(lvc_tile), (void **)&s->d_tiles)) {
-      s->tiles_cap = 0;
-      return 1;
-    }
+    lvc_set_error(ctx, "server tiles exceed their preallocated count");
+    return 1;
   }
   int tx = 0, ty = nx_t;
   s->ylds = 0;
@@ -277,8 +373,9 @@ static int admit(struct lvc_server *s) {
   int slot = 0;
   while (s->qhead < s->qtail && s->nresident < s->max_slots) {
     const queued_req *qr = &s->queue[s->qhead];
-    const int64_t bytes = (int64_t)request_bytes(s, &qr->req);
-    if (s->budget > 0 && s->resident_bytes + bytes > s->budget)
+    const int64_t len = request_elems(qr->req.numX, qr->req.numY);
+    const int64_t off = arena_take(s, len);
+    if (off < 0)
       break; /* in-order admission: wait for memory */
     while (s->slot_id[slot] >= 0)
       slot++;
@@ -295,8 +392,9 @@ static int admit(struct lvc_server *s) {
     q.beta = qr->req.beta;
     q.k = -1;
     lvc_grid_scalars(&q, 0);
-    q.state_off = (int64_t)slot * s->slot_elems;
+    q.state_off = off;
     q.geo_off = (int64_t)slot * s->slot_geo;
+    s->slot_len[slot] = len;
     q.fac_src = slot;
     {
       int gy, cpl, cols;
@@ -307,7 +405,7 @@ static int admit(struct lvc_server *s) {
     s->slot_id[slot] = qr->id;
     s->steps_left[slot] = q.nt - 1;
     s->results[qr->id].state = 1;
-    s->resident_bytes += bytes;
+    s->resident_bytes += request_bytes(s, q.nx, q.ny);
     s->nresident++;
     s->qhead++;
     s->tiles_dirty = 1;
@@ -370,9 +468,9 @@ int32_t lvc_server_step(struct lvc_server *s) {
         const int64_t id = s->slot_id[i];
         s->results[id].state = 2;
         s->results[id].price = s->h_out[i];
-        s->resident_bytes -= (int64_t)request_bytes(
-            s, &(struct lvc_request){s->h_slots[i].nx, s->h_slots[i].ny,
-                                     s->h_slots[i].nt, 0, 0, 0, 0, 0, 0});
+        s->resident_bytes -= request_bytes(s, s->h_slots[i].nx, s->h_slots[i].ny);
+        arena_give(s, s->h_slots[i].state_off, s->slot_len[i]);
+        s->slot_len[i] = 0;
         s->slot_id[i] = -1;
         s->h_slots[i].k = -1;
         s->nresident--;
@@ -412,6 +510,10 @@ int32_t lvc_server_drain(struct lvc_server *s) {
     }
   }
   return 0;
+}
+
+int64_t lvc_server_device_bytes(struct lvc_server *s) {
+  return s ? s->device_bytes : -1;
 }
 
 void lvc_server_stats(struct lvc_server *s, int64_t *queued, int64_t *resident,

@@ -124,6 +124,7 @@ _SIGNATURES = {
     "lvc_server_poll": (_i32, [_vp, _i64, ctypes.POINTER(_f64)]),
     "lvc_server_drain": (_i32, [_vp]),
     "lvc_server_stats": (None, [_vp] + [ctypes.POINTER(_i64)] * 4),
+    "lvc_server_device_bytes": (_i64, [_vp]),
 }
 
 
@@ -448,6 +449,10 @@ class Server:
     def drain(self) -> None:
         if self._L.lvc_server_drain(self._s) != 0:
             raise LocVolCalibError(f"lvc_server_drain: {self._ctx._error()}")
+
+    def device_bytes(self) -> int:
+        """Device memory the server holds (slot tables + grid arena)."""
+        return int(self._L.lvc_server_device_bytes(self._s))
 
     def stats(self) -> dict:
         vals = [_i64() for _ in range(4)]

@@ -91,9 +91,13 @@ struct lvc_request {
   double s0, strike, t, alpha, nu, beta;
 };
 
-/* max_slots: resident requests at most; mem_budget_bytes: device memory the
- * resident grids may use (0 = no limit besides max_slots); precision as
- * "lvc.precision". */
+/* max_slots: resident requests at most (grids up to 512 x 512);
+ * mem_budget_bytes: device memory the server may hold, all of it allocated
+ * here: per-slot tables (descriptor, geometry, y factor table, tiles: ~52 KB
+ * per slot in FP32) and, from the rest, one arena in which each resident
+ * request's two grids take an extent of their own size (a request waits in
+ * the queue until the arena has room); 0 = an arena for max_slots grids of
+ * 512 x 512.  precision as "lvc.precision". */
 struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots,
                                   int64_t mem_budget_bytes, int32_t precision);
 void lvc_server_free(struct lvc_server *srv);
@@ -109,7 +113,10 @@ int32_t lvc_server_step(struct lvc_server *srv);
 int32_t lvc_server_poll(struct lvc_server *srv, int64_t id, double *price);
 /* Ticks until nothing is queued or resident.  Returns 0 or -1. */
 int32_t lvc_server_drain(struct lvc_server *srv);
-/* Counters: queued, resident, completed, resident bytes. */
+/* Device bytes the server holds (slot tables + grid arena), <= the budget
+ * when one was given; -1 for NULL. */
+int64_t lvc_server_device_bytes(struct lvc_server *srv);
+/* Counters: queued, resident, completed, bytes of the resident grids. */
 void lvc_server_stats(struct lvc_server *srv, int64_t *queued, int64_t *resident,
                       int64_t *completed, int64_t *resident_bytes);
 

@@ -134,6 +134,43 @@ def test_cfg5_served_replay_vs_oracle_and_alone():
     assert np.array_equal(np.array(alone), mixed)
 
 
+def test_served_under_a_small_budget():
+    """64 slots but an 8 MiB budget (64 grids of 512x512 would take 128 MiB):
+    the server holds no more device memory than the budget, admits requests as
+    the arena has room, and every result equals the oracle's and the same
+    request's served alone (bit for bit)."""
+    n, budget = 96, 8 << 20
+    pr = uneven(n, 21, nt=(8, 24))
+    reqs = [_requests(pr, i) for i in range(n)]
+    say("oracle: 96 requests")
+    ref = oracle_batch(pr)
+    say("case: 96 requests, 64 slots, 8 MiB budget")
+    with lvc.LocVolCalib() as ctx:
+        srv = lvc.Server(ctx, max_slots=64, mem_budget=budget)
+        dev = srv.device_bytes()
+        assert dev <= budget
+        ids = [srv.submit(*r) for r in reqs]
+        peak_res, peak_bytes = 0, 0
+        while srv.stats()["resident"] or srv.stats()["queued"]:
+            srv.step()
+            st = srv.stats()
+            peak_res = max(peak_res, st["resident"])
+            peak_bytes = max(peak_bytes, st["resident_bytes"])
+        mixed = np.array([srv.poll(r) for r in ids])
+        srv.free()
+        say(f"   device bytes {dev}, peak resident {peak_res} requests / {peak_bytes} bytes")
+        assert peak_bytes <= dev and 1 < peak_res < 64    # memory, not slots, bound it
+        check(mixed, ref, lvc.PRECISION_FP32_REFINE)
+        srv = lvc.Server(ctx, max_slots=1)
+        alone = []
+        for r in reqs[:24]:
+            rid = srv.submit(*r)
+            srv.drain()
+            alone.append(srv.poll(rid))
+        srv.free()
+    assert np.array_equal(np.array(alone), mixed[:24])
+
+
 # ---- numX up to 1024, every x segment length, all precisions ----------------
 
 # numX sets per x segment length: 4 keeps <= 64 lanes up to 256 points, 8 up

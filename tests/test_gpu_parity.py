@@ -417,12 +417,19 @@ def test_server_alone_vs_mixed():
 
 
 def test_server_memory_budget():
+    """The budget bounds the server's device memory: slot tables first, the
+    grid arena from the rest; a request waits until the arena has room."""
     say("case: server memory budget")
     with lvc.LocVolCalib() as ctx:
-        # two f32 grids + geometry (FP64 x, taps; FP32 taps) of a 64x64 request
-        one = 2 * 64 * 64 * 4 + 8 * (6 * 64 + 2 * 64)
-        srv = lvc.Server(ctx, max_slots=8, mem_budget=2 * one + 10)
+        srv = lvc.Server(ctx, max_slots=8)             # no budget: 8 grids of 512x512
+        fixed = srv.device_bytes() - 8 * 2 * 512 * 512 * 4
+        srv.free()
+        assert fixed > 0
+        one = 2 * 64 * 64 * 4                          # R and Y of a 64x64 request
+        budget = fixed + 2 * one + 10
+        srv = lvc.Server(ctx, max_slots=8, mem_budget=budget)
         try:
+            assert srv.device_bytes() <= budget
             ids = [srv.submit(64, 64, 10, 0.03, 0.001 * i, 5.0, 0.2, 0.6, 0.5)
                    for i in range(5)]
             srv.step()
@@ -435,6 +442,8 @@ def test_server_memory_budget():
                 srv.submit(256, 256, 10, 0.03, 0.0, 5.0, 0.2, 0.6, 0.5)
         finally:
             srv.free()
+        with pytest.raises(lvc.LocVolCalibError, match="does not cover"):
+            lvc.Server(ctx, max_slots=8, mem_budget=fixed // 2)
 
 
 def test_profiling_report():
