@@ -573,6 +573,118 @@ int futhark_entry_main_f64(struct futhark_context *ctx,
   return 0;
 }
 
+void lvc_context_ready(struct futhark_context *ctx) {
+  lvc_tune_load(ctx);
+  /* the device is current here (futhark_context_new selected it) */
+  ctx->scratch_err = lvc_scratch_address(ctx->device, &ctx->scratch);
+}
+
+static int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
+
+/* futhark_entry_main without device allocation: the same options, segment
+ * lengths, tiles and kernels as futhark_entry_main (so the same bits), with
+ * every buffer carved from the static device scratch and the descriptors and
+ * tiles written there by a kernel; prices into the caller's device array. */
+int futhark_entry_main_noalloc(struct futhark_context *ctx, float *out,
+                               const int32_t outer, const int32_t numX,
+                               const int32_t numY, const int32_t numT,
+                               const float s0, const float t, const float alpha,
+                               const float nu, const float beta) {
+  if (!ctx)
+    return 1;
+  if (outer < 0) {
+    lvc_set_error(ctx, "outer_loop_count must be non-negative (got %d)", outer);
+    return 1;
+  }
+  if (lvc_validate_sizes(ctx, numX, numY, numT) || !lvc_ready(ctx) ||
+      check_main_index(ctx, outer, numX, numY, s0, t, alpha, nu, 1))
+    return 1;
+  if (outer == 0)
+    return 0;
+  if (!out) {
+    lvc_set_error(ctx, "main_noalloc: NULL output array");
+    return 1;
+  }
+  if (!ctx->scratch) {
+    lvc_set_error(ctx, "main_noalloc: the static device scratch of device %d is unavailable "
+                       "(%s)", ctx->device, hipGetErrorString((hipError_t)ctx->scratch_err));
+    return 1;
+  }
+  const int precision = ctx->cfg ? ctx->cfg->precision : LVC_FP32_REFINE;
+  const int64_t real = precision == LVC_FP64 ? 8 : 4;
+  int sx = ctx->cfg ? ctx->cfg->seg_x : 0, sy = ctx->cfg ? ctx->cfg->seg_y : 0, tx = 0, ty = 0;
+  if (!sx || !sy)
+    lvc_tune_peek(ctx, precision, numX, numY, &tx, &ty); /* never times (that allocates) */
+  sx = fit_seg(numX, sx ? sx : tx, 0);
+  sy = fit_seg(numY, sy ? sy : ty, 1);
+
+  lvc_problem tmpl;
+  memset(&tmpl, 0, sizeof(tmpl));
+  fill_main(&tmpl, 1, numX, numY, numT, s0, t, alpha, nu, beta, 1);
+  lvc_grid_scalars(&tmpl, 1);
+  lvc_main_layout lay;
+  memset(&lay, 0, sizeof(lay));
+  lay.outer = outer;
+  lay.gx = lvc_lanes_for(numX, sx);
+  lay.rows = LVC_BLOCK / lay.gx;
+  lay.xtpg = (numY + lay.rows - 1) / lay.rows;
+  lvc_ytile_shape(numY, sy, (int)real, &lay.gy, &lay.ycpl, &lay.cols);
+  lay.ytpo = (numX + lay.cols - 1) / lay.cols;
+  int xshare = lvc_xshare(ctx, precision);
+  if (xshare <= 0) {
+    const int runs[1] = {outer}, tiles[1] = {lay.xtpg};
+    xshare = batch_xshare(ctx, precision, 1, runs, tiles);
+  }
+  lay.xshare = xshare;
+  lay.groups = (outer + xshare - 1) / xshare;
+  lay.strike_f32 = 1;
+  lay.geo_stride = align64(LVC_GEO_LEN(numX, numY));
+  lay.state_stride = align64((int64_t)numX * numY);
+  const int64_t nxt = (int64_t)lay.groups * lay.xtpg, nyt = (int64_t)outer * lay.ytpo;
+  const int col = lvc_seg_is_col(sy);
+
+  /* the scratch, in order: descriptors, tiles, leaders, geometry, R, Y, y factors */
+  const int64_t o_tiles = align256((int64_t)outer * (int64_t)sizeof(lvc_problem));
+  const int64_t o_lead = o_tiles + align256((nxt + nyt) * (int64_t)sizeof(lvc_tile));
+  const int64_t o_geo = o_lead + align256((int64_t)lay.groups * 4);
+  const int64_t o_R = o_geo + align256((int64_t)outer * lay.geo_stride * 8);
+  const int64_t o_Y = o_R + align256((int64_t)outer * lay.state_stride * real);
+  const int64_t o_fac = o_Y + align256((int64_t)outer * lay.state_stride * real);
+  const int64_t need = o_fac + (col ? align256((int64_t)outer * LVC_YFAC_LEN * real) : 0);
+  if (need > LVC_SCRATCH_BYTES) {
+    lvc_set_error(ctx, "main_noalloc: main(%d, %d, %d, %d) needs %lld bytes of device scratch; "
+                       "the static scratch holds %lld",
+                  outer, numX, numY, numT, (long long)need, (long long)LVC_SCRATCH_BYTES);
+    return 1;
+  }
+  char *base = (char *)ctx->scratch;
+  lvc_problem *probs = (lvc_problem *)base;
+  lvc_tile *tl = (lvc_tile *)(base + o_tiles);
+  int32_t *lead = (int32_t *)(base + o_lead);
+  double *geo = (double *)(base + o_geo);
+  void *R = base + o_R, *Y = base + o_Y, *yfac = col ? base + o_fac : NULL;
+  const int ylds = lvc_ytile_lds(numY, sy, (int)real);
+  const int xshared = (xshare < outer ? xshare : outer) > 1;
+
+  void *st = (void *)ctx->stream;
+  pthread_mutex_lock(&ctx->lock);
+  int e = lvc_launch_main_init(st, &tmpl, &lay, probs, tl, lead);
+  if (!e)
+    e = lvc_launch_setup(st, precision, probs, outer, geo, R);
+  for (int s = 0; !e && s < numT - 1; s++) {
+    e = lvc_launch_xsweep(st, precision, sx, probs, tl, (int)nxt, geo, R, Y, yfac,
+                          col ? sy : 0, lead, lay.groups, xshared);
+    if (!e)
+      e = lvc_launch_ysweep(st, precision, sy, probs, tl + nxt, (int)nyt, geo, Y, R, ylds,
+                            yfac);
+  }
+  if (!e)
+    e = lvc_launch_gather(st, precision, probs, outer, R, out, 0);
+  const int rc = lvc_check_hip(ctx, e, "main_noalloc kernel launch");
+  pthread_mutex_unlock(&ctx->lock);
+  return rc;
+}
+
 static int download(struct futhark_context *ctx, void *dst, const void *src,
                     size_t bytes) {
   if (bytes == 0)

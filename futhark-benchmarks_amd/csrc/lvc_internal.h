@@ -78,6 +78,38 @@ typedef struct lvc_tile {
 #define LVC_YFAC_LEN (6 * LVC_COL_ROWS + 6 * (LVC_COL_BLOCK / 64))
 #define LVC_MAX_N 1024 /* largest numX / numY accepted (64 lanes x 16) */
 
+/* Layout of the reference's `main` (outer options sharing everything but the
+ * strike) in one contiguous device region, as the allocation-free entry
+ * builds it: descriptors, x tiles (groups of `xshare` options, `xtpg` row
+ * tiles each) then y tiles (`ytpo` per option), factor-group leaders,
+ * geometry, R, Y and the y factor tables.  The same tiles, in the same order,
+ * as the batch engine builds for those options. */
+typedef struct lvc_main_layout {
+  int32_t outer, xshare, groups;
+  int32_t xtpg, rows, gx;        /* x: tiles per group, rows per tile, lanes per row */
+  int32_t ytpo, cols, gy, ycpl;  /* y: tiles per option, columns per tile, lanes
+                                    (segments) per column, columns per lane (groups) */
+  int32_t strike_f32;            /* strikes 0.001f*i in f32 (else 0.001*i in f64) */
+  int32_t pad_;
+  int64_t geo_stride, state_stride; /* per option, in doubles / in reals */
+} lvc_main_layout;
+
+/* Bytes of the static device scratch of the allocation-free entry: main of
+ * 128 options at 256x256 in the FP32 forms (R and Y 64 MiB, geometry, factor
+ * tables, descriptors and tiles), or of 64 options in FP64. */
+#define LVC_SCRATCH_BYTES ((int64_t)72 << 20)
+
+/* Device address of `device`'s copy of the static scratch array, looked up
+ * once per device with that device current (hipGetSymbolAddress) and kept.
+ * Returns 0 or a hipError_t value. */
+int lvc_scratch_address(int device, void **ptr);
+
+/* Writes main's descriptors (each `tmpl` with its strike, offsets and factor
+ * leader), tiles and factor-group leaders for layout `lay`. */
+int lvc_launch_main_init(void *stream, const lvc_problem *tmpl,
+                         const lvc_main_layout *lay, lvc_problem *probs,
+                         lvc_tile *tiles, int32_t *fprobs);
+
 /* initGrid's scalars of one problem (LocVolCalib.fut:15-20) into q->dx, dy,
  * x0log, xindex, yindex and q->status (3 when myXindex is outside the grid,
  * the reference's failing bounds check at :236).  grid32 = 1 evaluates

@@ -52,7 +52,15 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "lvc_internal.h"
+
+// Device scratch of the allocation-free entry (futhark_entry_main_noalloc):
+// one copy per GPU, held until the process ends.  Host code never uses this
+// name as an address (on the host it names a stand-in, not device memory):
+// it takes the device address from lvc_scratch_address().
+static __device__ __align__(256) unsigned char scratch[LVC_SCRATCH_BYTES];
 
 namespace {
 
@@ -1102,6 +1110,42 @@ __global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
   out[p] = (O)R[pr.state_off + (int64_t)pr.yindex * pr.nx + pr.xindex];
 }
 
+// main's descriptors, tiles and factor leaders (lvc_main_layout): thread i
+// writes descriptor i, x tile i, y tile i and leader i where they exist.
+__global__ __launch_bounds__(256) void lvc_main_init_kernel(const lvc_problem tmpl,
+                                                            const lvc_main_layout lay,
+                                                            lvc_problem *probs, lvc_tile *tiles,
+                                                            int32_t *fprobs) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nxt = (int64_t)lay.groups * lay.xtpg, nyt = (int64_t)lay.outer * lay.ytpo;
+  if (i < lay.outer) {
+    const int p = (int)i;
+    lvc_problem q = tmpl;
+    // LocVolCalib.fut:247, strike_i = 0.001 * i (in f32 for the f32 entry)
+    q.strike = lay.strike_f32 ? (double)(0.001f * (float)p) : 0.001 * (double)p;
+    q.geo_off = (int64_t)p * lay.geo_stride;
+    q.state_off = (int64_t)p * lay.state_stride;
+    q.k = -1;
+    q.ydone = 0;
+    q.ytiles = lay.ytpo;
+    q.fac_src = (p / lay.xshare) * lay.xshare;
+    probs[p] = q;
+  }
+  if (i < lay.groups) fprobs[i] = (int32_t)i * lay.xshare;
+  if (i < nxt) {
+    const int g = (int)(i / lay.xtpg), first = (int)(i % lay.xtpg) * lay.rows;
+    const lvc_tile t = {g * lay.xshare, first, min(lay.rows, tmpl.ny - first), lay.gx,
+                        min(lay.xshare, lay.outer - g * lay.xshare), 0};
+    tiles[i] = t;
+  }
+  if (i < nyt) {
+    const int p = (int)(i / lay.ytpo), first = (int)(i % lay.ytpo) * lay.cols;
+    const lvc_tile t = {p, first, min(lay.cols, tmpl.nx - first), lay.gy, lay.ycpl, 0};
+    tiles[nxt + i] = t;
+  }
+}
+
 template <typename T, int M, bool REF>
 int launch_sweeps(hipStream_t st, bool xs, const lvc_problem *probs,
                   const lvc_tile *tiles, int ntiles, const double *geo,
@@ -1174,6 +1218,43 @@ int dispatch(hipStream_t st, bool xs, int precision, int seg,
 }  // namespace
 
 extern "C" {
+
+int lvc_scratch_address(int device, void **ptr) {
+  // one lookup per device, kept for the life of the process: every GPU holds
+  // its own copy of the array at its own address
+  constexpr int kMaxDevices = 64;
+  static void *addr[kMaxDevices];
+  static std::mutex mu;
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> guard(mu);
+  if (!addr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return (int)e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return (int)e;
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return (int)e;
+    if (!p) return (int)hipErrorInvalidSymbol;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return 0;
+}
+
+int lvc_launch_main_init(void *stream, const lvc_problem *tmpl,
+                         const lvc_main_layout *lay, lvc_problem *probs,
+                         lvc_tile *tiles, int32_t *fprobs) {
+  const int64_t n = max(max((int64_t)lay->outer, (int64_t)lay->groups * lay->xtpg),
+                        (int64_t)lay->outer * lay->ytpo);
+  if (n <= 0) return 0;
+  if (lay->xshare < 1 || lay->xtpg < 1 || lay->ytpo < 1) return (int)hipErrorInvalidValue;
+  lvc_main_init_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      *tmpl, *lay, probs, tiles, fprobs);
+  return (int)hipGetLastError();
+}
 
 int lvc_seg_supported(int seg, int ysweep) {
   return seg == 4 || seg == 8 || seg == 16 || (ysweep && (seg == 32 || seg == 64));

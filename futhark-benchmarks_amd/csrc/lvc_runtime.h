@@ -84,6 +84,11 @@ struct futhark_context {
    * gather read, from the device descriptor of its first problem */
   int dbg_index_valid;
   int32_t dbg_index[2];
+  /* device address of the static scratch of the allocation-free entry
+   * (looked up when the context opens its device), or NULL with the
+   * hipError_t of the failed lookup */
+  void *scratch;
+  int scratch_err;
 };
 
 /* What differs between the programs built on this runtime: the tuning
@@ -129,6 +134,10 @@ void lvc_prof_collect(struct futhark_context *ctx);
 /* tuning (lvc_tune.c) */
 void lvc_tune_pick(struct futhark_context *ctx, int precision, int nx, int ny,
                    int *seg_x, int *seg_y);
+/* the tuned choice without timing anything: the table (cache file, earlier
+ * tuning) or the built-in choice */
+void lvc_tune_peek(struct futhark_context *ctx, int precision, int nx, int ny,
+                   int *seg_x, int *seg_y);
 void lvc_tune_load(struct futhark_context *ctx);
 
 /* engine (lvc_engine.c) */
@@ -168,5 +177,7 @@ int lvc_validate_sizes(struct futhark_context *ctx, int64_t nx, int64_t ny,
 void lvc_choose_segs(struct futhark_context *ctx, int precision, int nx, int ny,
                      int *seg_x, int *seg_y);
 int lvc_xshare(struct futhark_context *ctx, int precision);
+/* LocVolCalib's context-ready hook: the tuning cache and the scratch address */
+void lvc_context_ready(struct futhark_context *ctx);
 
 #endif /* LVC_RUNTIME_H */

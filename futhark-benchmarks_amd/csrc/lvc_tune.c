@@ -134,6 +134,17 @@ static double time_candidate(struct futhark_context *ctx, int precision, int nx,
   return best;
 }
 
+void lvc_tune_peek(struct futhark_context *ctx, int precision, int nx, int ny,
+                   int *seg_x, int *seg_y) {
+  int i = table_find(ctx, precision, nx, ny);
+  if (i >= 0) {
+    *seg_x = ctx->tuned[i].seg_x;
+    *seg_y = ctx->tuned[i].seg_y;
+    return;
+  }
+  builtin(precision, nx, ny, seg_x, seg_y);
+}
+
 void lvc_tune_pick(struct futhark_context *ctx, int precision, int nx, int ny,
                    int *seg_x, int *seg_y) {
   int i = table_find(ctx, precision, nx, ny);
@@ -221,4 +232,5 @@ static int set_param(struct futhark_context_config *cfg, const char *name, size_
 }
 
 const lvc_program lvc_prog = {(int)(sizeof(tuning_names) / sizeof(tuning_names[0])),
-                              tuning_names, tuning_classes, set_param, lvc_tune_load};
+                              tuning_names, tuning_classes, set_param,
+                              lvc_context_ready};

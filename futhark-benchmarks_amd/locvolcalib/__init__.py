@@ -95,6 +95,8 @@ _SIGNATURES = {
     "futhark_entry_main_f64": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _i32, _i32, _i32, _i32,
                                               _f64, _f64, _f64, _f64, _f64]),
     "futhark_entry_main_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)] + [_vp] * 9 + [_i32]),
+    "futhark_entry_main_noalloc": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32,
+                                                  _f32, _f32, _f32, _f32, _f32]),
     "futhark_new_f32_2d": (_vp, [_vp, _vp, _i64, _i64]),
     "futhark_new_raw_f32_2d": (_vp, [_vp, _vp, _i64, _i64]),
     "futhark_free_f32_2d": (ctypes.c_int, [_vp, _vp]),
@@ -308,6 +310,25 @@ class LocVolCalib:
         h = self.main_raw(outer, numX, numY, numT, s0, t, alpha, nu, beta)
         try:
             return self._values("f32", h)
+        finally:
+            self._free("f32", h)
+
+    def main_noalloc_raw(self, out_ptr: int, outer, numX, numY, numT, s0, t, alpha, nu,
+                         beta) -> None:
+        """Enqueue ``futhark_entry_main_noalloc``: prices into the device
+        array at ``out_ptr`` (``outer`` floats), no device allocation."""
+        self._check(self._L.futhark_entry_main_noalloc(self._ctx, out_ptr, outer, numX, numY,
+                                                       numT, s0, t, alpha, nu, beta),
+                    "futhark_entry_main_noalloc")
+
+    def main_noalloc(self, outer, numX, numY, numT, s0, t, alpha, nu, beta) -> np.ndarray:
+        """``main`` through the allocation-free entry: the output is a
+        library array made here beforehand, its device memory passed raw."""
+        h = self._new("f32", np.zeros(max(int(outer), 1), np.float32))
+        try:
+            ptr = self._L.futhark_values_raw_f32_1d(self._ctx, h)
+            self.main_noalloc_raw(ptr, outer, numX, numY, numT, s0, t, alpha, nu, beta)
+            return self._values("f32", h)[:max(int(outer), 0)]
         finally:
             self._free("f32", h)
 

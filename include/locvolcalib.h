@@ -56,6 +56,28 @@ int futhark_entry_main_f64(struct futhark_context *ctx,
                            const double s0, const double t, const double alpha,
                            const double nu, const double beta);
 
+/* futhark_entry_main without device memory allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same options, precision ("lvc.precision"), segment lengths and kernels as
+ * futhark_entry_main, so the same prices bit for bit; the prices go to
+ * `out`, a device array of `outer` floats owned by the caller (for example
+ * from futhark_values_raw_f32_1d), enqueued on the context's stream
+ * (futhark_context_sync waits for them).  Every buffer comes from one static
+ * device array of 72 MiB per GPU (held until the process ends): main(128,
+ * 256, 256, ...) in the FP32 forms, main(64, 256, 256, ...) in FP64; larger
+ * calls fail with a message giving the bytes they need.  Calls on one GPU
+ * share that array: calls of one context are ordered by its stream, but calls
+ * from two contexts on the same GPU that overlap in time race.  Segment
+ * lengths come from the cache file or the built-in table (the autotuner's
+ * timing allocates, so it is never run here); profiling does not cover these
+ * calls.  Errors as futhark_entry_main, plus NULL `out` and the size limit. */
+int futhark_entry_main_noalloc(struct futhark_context *ctx, float *out,
+                               const int32_t outer_loop_count, const int32_t numX,
+                               const int32_t numY, const int32_t numT,
+                               const float s0, const float t, const float alpha,
+                               const float nu, const float beta);
+
 /* Batched form: problem p has its own sizes (numX[p], numY[p], numT[p]) and
  * parameters and runs LocVolCalib.fut:211-236 (`value`).  All arrays have
  * the same length n.  `precision` as "lvc.precision".  The problems advance
