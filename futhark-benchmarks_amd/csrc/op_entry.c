@@ -101,14 +101,13 @@ int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0
                   "has %lld", contract_number, (long long)cK, (long long)K);
     return 1;
   }
-  if (B > 31) {
-    lvc_set_error(ctx, "main: %lld direction-vector bits (at most 31: the Sobol "
-                  "index is an i32, :12)", (long long)B);
+  if (B > OP_DV_STRIDE) {
+    lvc_set_error(ctx, "main: %lld direction-vector bits (at most 32: testBit shifts "
+                  "an i32, :17-18)", (long long)B);
     return 1;
   }
-  if (M > OP_MAX_MODELS) {
-    lvc_set_error(ctx, "main: %lld market models (at most %d)", (long long)M,
-                  OP_MAX_MODELS);
+  if (M > (1 << 24)) {
+    lvc_set_error(ctx, "main: %lld market models (at most 2^24)", (long long)M);
     return 1;
   }
   const int launch = priced && num_mc_it > 0 && M > 0;
@@ -142,10 +141,16 @@ int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0
   } else if (!rc && M > 0) {
     /* no payoff to sum (:294) or no path: the sum is 0, divided by
      * num_mc_it (:322), which makes NaN for no path */
-    float host[OP_MAX_MODELS];
-    for (int64_t m = 0; m < M; m++)
-      host[m] = 0.0f / (float)num_mc_it;
-    rc = lvc_upload(ctx, out, host, (size_t)M * sizeof(float));
+    float *host = malloc((size_t)M * sizeof(float));
+    if (!host) {
+      lvc_set_error(ctx, "host allocation failed");
+      rc = 1;
+    } else {
+      for (int64_t m = 0; m < M; m++)
+        host[m] = 0.0f / (float)num_mc_it;
+      rc = lvc_upload(ctx, out, host, (size_t)M * sizeof(float));
+      free(host);
+    }
   }
   if (partial)
     lvc_dev_free(ctx, partial, part_bytes);

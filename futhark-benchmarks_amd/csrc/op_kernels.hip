@@ -271,11 +271,11 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
   double sum[OP_MAX_MODELS];
 #pragma unroll
   for (int m = 0; m < OP_MAX_MODELS; ++m) sum[m] = 0.0;
-  // x / 2 ** num_bits (:31-35): for any i32 x and bits <= 31 the quotient
+  // x / 2 ** num_bits (:31-35): for any i32 x and bits <= 32 the quotient
   // is a power-of-two scaling of (float)x with no underflow, hence exact,
   // and so is the product with the exact reciprocal 2 ** -num_bits: same
   // bits, one multiply instead of an IEEE division
-  const float inv_divisor = 1.0f / (float)(1u << bits);
+  const float inv_divisor = ldexpf(1.0f, -bits);
   float rows[U][D];
   // blocks of kThreads Sobol indices covering 0 .. N (index 0 is no path)
   const int64_t nblk = N / kThreads + 1;
@@ -420,22 +420,28 @@ int op_launch(void *stream, int contract, const op_args *A, const int32_t *dir_v
               int32_t *dv, double *partial, float *out) {
   hipStream_t st = (hipStream_t)stream;
   const int dims = contract == 1 ? 1 : contract == 2 ? 15 : contract == 3 ? 1101 : 0;
-  if (dims == 0 || A->M < 1 || A->M > OP_MAX_MODELS || A->bits < 0 || A->bits > 31 ||
-      A->N < 1)
+  if (dims == 0 || A->M < 1 || A->bits < 0 || A->bits > OP_DV_STRIDE || A->N < 1)
     return (int)hipErrorInvalidValue;
   const int np = dims * OP_DV_STRIDE;
   op_pad_kernel<<<(np + 255) / 256, 256, 0, st>>>(dir_vs, dv, dims, A->bits);
   const int nblocks = op_blocks(A->N);
-#define OP_PATHS_ARGS                                                              \
-  dv, A->bbi, A->bbd, A->cs, A->vols, A->drifts, A->sts, A->detvals, A->discts, \
-      A->bits, A->K, A->M, A->N, partial
-  switch (contract) {
-    case 1: op_paths_kernel<1><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
-    case 2: op_paths_kernel<2><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
-    default: op_paths_kernel<3><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
-  }
+  const int D = contract == 1 ? 1 : contract == 2 ? 5 : 367, U = contract == 1 ? 1 : 3;
+  // the models in chunks of OP_MAX_MODELS (a lane's sums are registers): each
+  // chunk reruns the paths with its models' market data and writes its means
+  for (int m0 = 0; m0 < A->M; m0 += OP_MAX_MODELS) {
+    const int mc = A->M - m0 < OP_MAX_MODELS ? A->M - m0 : OP_MAX_MODELS;
+#define OP_PATHS_ARGS                                                                   \
+  dv, A->bbi, A->bbd, A->cs + (int64_t)m0 * U * U, A->vols + (int64_t)m0 * D * U,       \
+      A->drifts + (int64_t)m0 * D * U, A->sts + (int64_t)m0 * U, A->detvals + m0,       \
+      A->discts + (int64_t)m0 * A->K, A->bits, A->K, mc, A->N, partial
+    switch (contract) {
+      case 1: op_paths_kernel<1><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
+      case 2: op_paths_kernel<2><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
+      default: op_paths_kernel<3><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
+    }
 #undef OP_PATHS_ARGS
-  op_mean_kernel<<<1, kThreads, 0, st>>>(partial, nblocks, A->M, A->N, out);
+    op_mean_kernel<<<1, kThreads, 0, st>>>(partial, nblocks, mc, A->N, out + m0);
+  }
   return (int)hipGetLastError();
 }
 

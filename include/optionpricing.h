@@ -35,8 +35,11 @@ extern "C" {
  * md_detvals [num_models][1], md_discts [num_models][num_discts], bb_inds
  * and bb_data [3][num_dates]; out0 [num_models].  Contract 1 takes
  * num_dates = num_und = 1, contract 2 [5][3], contract 3 [367][3] (the
- * reference's size coercions at :289-293); another shape, another contract
- * number, num_mc_it < 1, num_bits > 31 or more than 8 models is an error. */
+ * reference's size coercions at :289-293); another shape for contracts
+ * 1-3, num_mc_it < 0 or num_bits > 32 (testBit shifts an i32, :17-18) is an
+ * error.  As the reference: a contract number other than 1-3 prices 0 for
+ * every model (:294), num_mc_it = 0 gives NaN (0/0, :322), no model gives
+ * an empty array.  Any number of models (taken 8 at a time on the device). */
 int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0,
                        const int32_t contract_number, const int32_t num_mc_it,
                        const struct futhark_i32_2d *dir_vs,

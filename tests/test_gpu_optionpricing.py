@@ -85,7 +85,8 @@ def _models(args, m, seed):
             np.repeat(detv, m, 0), scale(np.repeat(disc, m, 0), 0.01), bbi, bbd)
 
 
-@pytest.mark.parametrize("name,m", [("small", 8), ("medium", 3), ("large", 2)])
+@pytest.mark.parametrize("name,m", [("small", 8), ("medium", 3), ("large", 2), ("small", 17),
+                                    ("medium", 11), ("large", 9)])
 def test_several_models(ctx, datasets, name, m):
     args = _models(datasets[name][0], m, seed=m)
     n = 20000
@@ -104,10 +105,13 @@ def test_fewer_direction_bits(ctx, datasets, bits):
     np.testing.assert_allclose(got, oracle.op_main(args[0], n, dv, *args[3:]), rtol=RTOL)
 
 
-def test_31_direction_bits(ctx, datasets):
+@pytest.mark.parametrize("bits", [31, 32])
+def test_full_direction_bits(ctx, datasets, bits):
+    """31 and 32 direction bits (the most testBit can address in an i32,
+    :17-18; Gray-code bit 31 of a non-negative index is never set)."""
     args, _ = datasets["medium"]
-    rng = np.random.default_rng(31)
-    dv = rng.integers(0, 1 << 31, size=(15, 31), dtype=np.int64).astype(np.int32)
+    rng = np.random.default_rng(bits)
+    dv = rng.integers(-(1 << 31), 1 << 31, size=(15, bits), dtype=np.int64).astype(np.int32)
     dv[:, 30] = 1 << 30
     n = 5000
     got = ctx.main(args[0], n, dv, *args[3:])
@@ -204,8 +208,7 @@ def test_no_models(ctx, datasets):
 @pytest.mark.parametrize("case,fragment", [
     ("contract", "contract 1 takes"),
     ("negative", "negative"),
-    ("bits", "at most 31"),
-    ("models", "at most 8"),
+    ("bits", "at most 32"),
     ("discounts", "discounts"),
     ("shape", "md_vols"),
 ])
@@ -217,9 +220,7 @@ def test_errors(ctx, datasets, case, fragment):
     elif case == "negative":
         n = -1
     elif case == "bits":
-        dv = np.zeros((15, 32), np.int32)
-    elif case == "models":
-        c, n, dv, cs, vols, drifts, sts, detv, disc, bbi, bbd = _models(args, 9, 0)
+        dv = np.zeros((15, 33), np.int32)
     elif case == "discounts":
         disc = disc[:, :4]
     elif case == "shape":
