@@ -228,17 +228,29 @@ def bench_batched(lvc, reps=3, n=512, seed=42, check=True):
     """512 uneven problems in one batch call (FP32 + refinement), and its
     errors against the FP64 oracle."""
     import numpy as np
+    import numpy as np
     pr = uneven_problems(n, seed)
-    with lvc.LocVolCalib(device=0) as ctx:
-        got = ctx.main_batch(**pr)                 # warm-up (and tuning)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            ctx.main_batch(**pr)
-        dt = (time.perf_counter() - t0) / reps
+    times = {}
+    for graph in (0, 1):
+        with lvc.LocVolCalib(device=0, tuning={"lvc.graph": graph}) as ctx:
+            got = ctx.main_batch(**pr)             # warm-up (tuning; the graph kept)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.main_batch(**pr)
+            times[graph] = (time.perf_counter() - t0) / reps
+    dt = times[0]
     elems = float((pr["numX"].astype(float) * pr["numY"] * (pr["numT"] - 1)).sum())
+    segs = {}
+    for nx, ny in zip(pr["numX"], pr["numY"]):
+        segs[f"{nx}x{ny}"] = segs.get(f"{nx}x{ny}", 0) + 1
     out = {"problems": n, "seed": seed, "ms_per_batch": round(dt * 1e3, 3),
            "problems_per_s": round(n / dt, 2),
-           "grid_point_steps_per_s": round(elems / dt, 1)}
+           "grid_point_steps_per_s": round(elems / dt, 1),
+           "graph_replay": {"ms_per_batch": round(times[1] * 1e3, 3),
+                            "problems_per_s": round(n / times[1], 2),
+                            "how": "the step launches replayed from the context's kept "
+                                   "HIP graph (lvc.graph), against direct launches"},
+           "size_classes": len(segs)}
     log(f"batched: {out['ms_per_batch']} ms per batch")
     if check:
         log("batched: FP64 oracle")

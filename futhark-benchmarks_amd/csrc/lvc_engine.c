@@ -327,75 +327,143 @@ void lvc_batch_release(struct futhark_context *ctx, lvc_batch *b) {
   memset(b, 0, sizeof(*b));
 }
 
-/* The step sequence (an x and a y launch per time step, each over the tiles
- * of the problems still running) captured into one HIP graph and launched
- * once.  Synchronous: the graph is destroyed when it has run. */
-static int run_graph(struct futhark_context *ctx, lvc_batch *b,
-                     const lvc_tile *xt, const lvc_tile *yt) {
+/* One time step of every batch in bs that still runs: the x sweeps, then
+ * the y sweeps (each batch is a set of problems sharing one segment pair). */
+static int launch_step(struct futhark_context *ctx, lvc_batch *bs, int nb, int64_t s) {
   void *st = (void *)ctx->stream;
-  hipGraph_t graph = NULL;
-  hipGraphExec_t exec = NULL;
-  int rc = lvc_check_hip(ctx, hipStreamBeginCapture(ctx->stream,
-                                                    hipStreamCaptureModeThreadLocal),
-                         "hipStreamBeginCapture");
-  if (rc)
-    return rc;
-  int e = 0;
-  for (int64_t s = 0; !e && s < b->steps; s++) {
-    e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
-                          b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
-                          lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
-                          b->f_active[s], b->xshared);
-    if (!e)
-      e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
-                            b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds,
-                            b->d_yfac);
-  }
-  const int e2 = (int)hipStreamEndCapture(ctx->stream, &graph);
-  rc = lvc_check_hip(ctx, e, "sweep kernel (capture)") ||
-       lvc_check_hip(ctx, e2, "hipStreamEndCapture") ||
-       lvc_check_hip(ctx, hipGraphInstantiate(&exec, graph, NULL, NULL, 0),
-                     "hipGraphInstantiate") ||
-       lvc_check_hip(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch") ||
-       lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  if (exec)
-    (void)hipGraphExecDestroy(exec);
-  if (graph)
-    (void)hipGraphDestroy(graph);
-  return rc;
-}
-
-int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
-  if (b->n == 0)
-    return 0;
-  void *st = (void *)ctx->stream;
-  int tok = lvc_prof_begin(ctx, "lvc_setup");
-  int e = lvc_launch_setup(st, b->precision, b->d_probs, b->n, b->d_geo, b->d_R);
-  lvc_prof_end(ctx, tok);
-  if (lvc_check_hip(ctx, e, "setup kernel"))
-    return 1;
-  const lvc_tile *xt = b->d_tiles, *yt = b->d_tiles + b->nxt;
-  const int profiling = ctx->cfg && ctx->cfg->profiling && !ctx->prof_paused;
-  if (b->graph && !profiling)
-    return run_graph(ctx, b, xt, yt);
-  for (int64_t s = 0; s < b->steps; s++) {
-    tok = lvc_prof_begin(ctx, "lvc_xsweep");
-    e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, xt,
-                          b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
-                          lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
-                          b->f_active[s], b->xshared);
+  for (int g = 0; g < nb; g++) {
+    lvc_batch *b = &bs[g];
+    if (s >= b->steps)
+      continue;
+    const int tok = lvc_prof_begin(ctx, "lvc_xsweep");
+    const int e = lvc_launch_xsweep(st, b->precision, b->seg_x, b->d_probs, b->d_tiles,
+                                    b->x_active[s], b->d_geo, b->d_R, b->d_Y, b->d_yfac,
+                                    lvc_seg_is_col(b->seg_y) ? b->seg_y : 0, b->d_fprobs,
+                                    b->f_active[s], b->xshared);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "x sweep kernel"))
       return 1;
-    tok = lvc_prof_begin(ctx, "lvc_ysweep");
-    e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs, yt,
-                          b->y_active[s], b->d_geo, b->d_Y, b->d_R, b->ylds,
-                          b->d_yfac);
+  }
+  for (int g = 0; g < nb; g++) {
+    lvc_batch *b = &bs[g];
+    if (s >= b->steps)
+      continue;
+    const int tok = lvc_prof_begin(ctx, "lvc_ysweep");
+    const int e = lvc_launch_ysweep(st, b->precision, b->seg_y, b->d_probs,
+                                    b->d_tiles + b->nxt, b->y_active[s], b->d_geo, b->d_Y,
+                                    b->d_R, b->ylds, b->d_yfac);
     lvc_prof_end(ctx, tok);
     if (lvc_check_hip(ctx, e, "y sweep kernel"))
       return 1;
   }
   return 0;
+}
+
+/* FNV-1a over the arguments of every launch of a step sequence: equal keys,
+ * equal launches. */
+static uint64_t fnv(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = p;
+  for (size_t i = 0; i < n; i++)
+    h = (h ^ c[i]) * 1099511628211ull;
+  return h;
+}
+
+static uint64_t launch_key(const lvc_batch *bs, int nb) {
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(h, &nb, sizeof(nb));
+  for (int g = 0; g < nb; g++) {
+    const lvc_batch *b = &bs[g];
+    const void *ptrs[7] = {b->d_probs, b->d_tiles, b->d_geo, b->d_R, b->d_Y, b->d_yfac,
+                           b->d_fprobs};
+    const int ints[8] = {b->precision, b->seg_x, b->seg_y, b->nxt, b->nyt, b->xshared,
+                         b->ylds, (int)b->steps};
+    h = fnv(h, ptrs, sizeof(ptrs));
+    h = fnv(h, ints, sizeof(ints));
+    h = fnv(h, b->x_active, (size_t)b->steps * sizeof(int));
+    h = fnv(h, b->y_active, (size_t)b->steps * sizeof(int));
+    h = fnv(h, b->f_active, (size_t)b->steps * sizeof(int));
+  }
+  return h;
+}
+
+/* The step sequence captured into a HIP graph and launched as one.  The
+ * context keeps the instantiated graph of its last sequence: a call whose
+ * launches are the same (sizes, active tiles and buffers: the memory pool
+ * hands a repeated call the same blocks) launches it again without capture;
+ * another sequence is captured and the kept graph updated in place
+ * (hipGraphExecUpdate), or instantiated anew when the update is refused. */
+static int run_graph(struct futhark_context *ctx, lvc_batch *bs, int nb, int64_t steps) {
+  const uint64_t key = launch_key(bs, nb);
+  if (!(ctx->graph_exec && ctx->graph_key == key)) {
+    hipGraph_t graph = NULL;
+    if (lvc_check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+                      "hipStreamBeginCapture"))
+      return 1;
+    int rc = 0;
+    for (int64_t s = 0; !rc && s < steps; s++)
+      rc = launch_step(ctx, bs, nb, s);
+    const int e = (int)hipStreamEndCapture(ctx->stream, &graph);
+    if (rc || lvc_check_hip(ctx, e, "hipStreamEndCapture")) {
+      if (graph)
+        (void)hipGraphDestroy(graph);
+      return 1;
+    }
+    int updated = 0;
+    if (ctx->graph_exec) {
+      hipGraphExecUpdateResult res;
+      hipGraphNode_t bad = NULL;
+      updated = hipGraphExecUpdate((hipGraphExec_t)ctx->graph_exec, graph, &bad, &res) ==
+                    hipSuccess && res == hipGraphExecUpdateSuccess;
+      if (!updated) {
+        /* a refused update (another topology) is not an error here: clear
+         * the thread's last error, which the next launch check would read */
+        (void)hipGetLastError();
+        lvc_graph_release(ctx);
+      }
+    }
+    if (!updated) {
+      hipGraphExec_t exec = NULL;
+      rc = lvc_check_hip(ctx, hipGraphInstantiate(&exec, graph, NULL, NULL, 0),
+                         "hipGraphInstantiate");
+      ctx->graph_exec = rc ? NULL : exec;
+    }
+    (void)hipGraphDestroy(graph);
+    if (rc)
+      return 1;
+    ctx->graph_key = key;
+  }
+  return lvc_check_hip(ctx, hipGraphLaunch((hipGraphExec_t)ctx->graph_exec, ctx->stream),
+                       "hipGraphLaunch");
+}
+
+/* Runs the batches in bs together: their setups, then the time steps, each
+ * step every batch's x sweep then every batch's y sweep. */
+static int run_batches(struct futhark_context *ctx, lvc_batch *bs, int nb) {
+  int64_t steps = 0;
+  for (int g = 0; g < nb; g++) {
+    lvc_batch *b = &bs[g];
+    if (b->n == 0)
+      continue;
+    const int tok = lvc_prof_begin(ctx, "lvc_setup");
+    const int e = lvc_launch_setup((void *)ctx->stream, b->precision, b->d_probs, b->n,
+                                   b->d_geo, b->d_R);
+    lvc_prof_end(ctx, tok);
+    if (lvc_check_hip(ctx, e, "setup kernel"))
+      return 1;
+    if (b->steps > steps)
+      steps = b->steps;
+  }
+  const int profiling = ctx->cfg && ctx->cfg->profiling && !ctx->prof_paused;
+  if (nb > 0 && bs[0].graph && !profiling && steps > 0)
+    return run_graph(ctx, bs, nb, steps);
+  for (int64_t s = 0; s < steps; s++)
+    if (launch_step(ctx, bs, nb, s))
+      return 1;
+  return 0;
+}
+
+int lvc_batch_run(struct futhark_context *ctx, lvc_batch *b) {
+  return run_batches(ctx, b, 1);
 }
 
 int lvc_batch_gather(struct futhark_context *ctx, lvc_batch *b, void *out,
@@ -412,38 +480,74 @@ int lvc_batch_gather(struct futhark_context *ctx, lvc_batch *b, void *out,
 static int download(struct futhark_context *ctx, void *dst, const void *src,
                     size_t bytes);
 
-/* Runs a batch end to end into a new device array of n prices. */
+/* Runs a batch end to end into a new device array of n prices.  Each
+ * problem takes its own size class's segment pair; the problems of each
+ * distinct pair form a sub-batch, the sub-batches advance together, and
+ * each problem's price goes to its own place in the output. */
+#define LVC_MAX_GROUPS 16
 static int run_batch(struct futhark_context *ctx, int n, const lvc_problem *probs,
                      int precision, int out_f64, void **out_mem) {
-  int nxm = 3, nym = 3;
-  for (int p = 0; p < n; p++) {
-    if (probs[p].nx > nxm)
-      nxm = probs[p].nx;
-    if (probs[p].ny > nym)
-      nym = probs[p].ny;
-  }
-  int sx, sy;
-  lvc_choose_segs(ctx, precision, nxm, nym, &sx, &sy);
-  lvc_batch b;
   const size_t ob = (size_t)n * (out_f64 ? sizeof(double) : sizeof(float));
-  if (lvc_dev_alloc(ctx, ob, out_mem))
+  int gsx[LVC_MAX_GROUPS], gsy[LVC_MAX_GROUPS], gcnt[LVC_MAX_GROUPS] = {0}, ng = 0;
+  int *grp = malloc((size_t)(n ? n : 1) * sizeof(int));
+  lvc_problem *sub = malloc((size_t)(n ? n : 1) * sizeof(lvc_problem));
+  if (!grp || !sub) {
+    free(grp);
+    free(sub);
+    lvc_set_error(ctx, "host allocation failed");
     return 1;
-  int rc = lvc_batch_build(ctx, &b, n, probs, precision, sx, sy);
+  }
+  for (int p = 0; p < n; p++) {
+    int sx, sy, g;
+    lvc_choose_segs(ctx, precision, probs[p].nx, probs[p].ny, &sx, &sy);
+    for (g = 0; g < ng && (gsx[g] != sx || gsy[g] != sy); g++)
+      ;
+    if (g == ng) {
+      if (ng == LVC_MAX_GROUPS) { /* cannot happen: 3 x 5 segment pairs */
+        free(grp);
+        free(sub);
+        lvc_set_error(ctx, "too many segment classes");
+        return 1;
+      }
+      gsx[ng] = sx;
+      gsy[ng] = sy;
+      ng++;
+    }
+    grp[p] = g;
+    gcnt[g]++;
+  }
+  lvc_batch bs[LVC_MAX_GROUPS];
+  memset(bs, 0, sizeof(bs));
+  int rc = lvc_dev_alloc(ctx, ob, out_mem);
+  for (int g = 0; !rc && g < ng; g++) {
+    int m = 0;
+    for (int p = 0; p < n; p++)
+      if (grp[p] == g) {
+        sub[m] = probs[p];
+        sub[m].out_index = p;
+        m++;
+      }
+    rc = lvc_batch_build(ctx, &bs[g], m, sub, precision, gsx[g], gsy[g]);
+  }
   if (!rc)
-    rc = lvc_batch_run(ctx, &b);
-  if (!rc)
-    rc = lvc_batch_gather(ctx, &b, *out_mem, out_f64);
+    rc = run_batches(ctx, bs, ng);
+  for (int g = 0; !rc && g < ng; g++)
+    rc = lvc_batch_gather(ctx, &bs[g], *out_mem, out_f64);
   if (!rc && n > 0 && ctx->cfg && ctx->cfg->debugging) {
-    /* the index the gather read: problem 0's device descriptor */
+    /* the index the gather read: problem 0's device descriptor (the first
+     * problem of the first sub-batch) */
     lvc_problem q;
-    rc = download(ctx, &q, b.d_probs, sizeof(q));
+    rc = download(ctx, &q, bs[0].d_probs, sizeof(q));
     if (!rc) {
       ctx->dbg_index[0] = q.yindex;
       ctx->dbg_index[1] = q.xindex;
       ctx->dbg_index_valid = 1;
     }
   }
-  lvc_batch_release(ctx, &b);
+  for (int g = 0; g < ng; g++)
+    lvc_batch_release(ctx, &bs[g]);
+  free(grp);
+  free(sub);
   if (rc) {
     lvc_dev_free(ctx, *out_mem, ob);
     *out_mem = NULL;

@@ -43,7 +43,8 @@ typedef struct lvc_problem {
   int32_t ydone;           /* y-sweep workgroups finished in this step */
   int32_t fac_src;         /* problem whose y factor table this one uses */
   int32_t grid32;          /* initGrid's scalars in f32 (the reference's f32 entry) */
-  int32_t pad_[2];
+  int32_t out_index;       /* where the gather writes this problem's price */
+  int32_t pad_;
   int64_t state_off;       /* element offset of R and Y */
   int64_t geo_off;         /* element offset into the f64 geometry buffer */
   double s0, strike, t, alpha, nu, beta;
@@ -144,6 +145,7 @@ int lvc_launch_ysweep(void *stream, int precision, int seg,
                       const lvc_problem *probs, const lvc_tile *tiles,
                       int ntiles, const double *geo, const void *Y, void *R,
                       int lds_bytes, const void *yfac);
+/* The price of each of the nprob problems into out[problem.out_index]. */
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
                       int nprob, const void *R, void *out, int out_f64);
 

@@ -1096,7 +1096,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   }
 }
 
-// value (:236): the price at (yindex, xindex).
+// value (:236): the price at (yindex, xindex), to the problem's output slot.
 template <typename T, typename O>
 __global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
                                   O *out) {
@@ -1104,10 +1104,10 @@ __global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
   if (p >= n) return;
   const lvc_problem &pr = probs[p];
   if (pr.status != 0) {
-    out[p] = (O)NAN;
+    out[pr.out_index] = (O)NAN;
     return;
   }
-  out[p] = (O)R[pr.state_off + (int64_t)pr.yindex * pr.nx + pr.xindex];
+  out[pr.out_index] = (O)R[pr.state_off + (int64_t)pr.yindex * pr.nx + pr.xindex];
 }
 
 // main's descriptors, tiles and factor leaders (lvc_main_layout): thread i
@@ -1130,6 +1130,7 @@ __global__ __launch_bounds__(256) void lvc_main_init_kernel(const lvc_problem tm
     q.ydone = 0;
     q.ytiles = lay.ytpo;
     q.fac_src = (p / lay.xshare) * lay.xshare;
+    q.out_index = p;
     probs[p] = q;
   }
   if (i < lay.groups) fprobs[i] = (int32_t)i * lay.xshare;

@@ -161,7 +161,15 @@ struct futhark_context *futhark_context_new(struct futhark_context_config *cfg) 
   return ctx;
 }
 
+void lvc_graph_release(struct futhark_context *ctx) {
+  if (ctx->graph_exec)
+    (void)hipGraphExecDestroy((hipGraphExec_t)ctx->graph_exec);
+  ctx->graph_exec = NULL;
+  ctx->graph_key = 0;
+}
+
 static void pool_release_all(struct futhark_context *ctx) {
+  lvc_graph_release(ctx);
   for (int i = 0; i < ctx->npool; i++)
     (void)hipFree(ctx->pool[i].ptr);
   ctx->npool = 0;

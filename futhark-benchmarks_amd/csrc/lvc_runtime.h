@@ -89,6 +89,10 @@ struct futhark_context {
    * hipError_t of the failed lookup */
   void *scratch;
   int scratch_err;
+  /* the instantiated HIP graph of the last step sequence replayed with
+   * "lvc.graph" (a hipGraphExec_t) and the key of its launches */
+  void *graph_exec;
+  uint64_t graph_key;
 };
 
 /* What differs between the programs built on this runtime: the tuning
@@ -107,6 +111,9 @@ extern const lvc_program lvc_prog;
 void lvc_set_error(struct futhark_context *ctx, const char *fmt, ...);
 int lvc_check_hip(struct futhark_context *ctx, int err, const char *what);
 int lvc_ready(struct futhark_context *ctx);
+
+/* drops the kept graph (its launches name pool memory) */
+void lvc_graph_release(struct futhark_context *ctx);
 
 /* device memory pool */
 int lvc_dev_alloc(struct futhark_context *ctx, size_t bytes, void **ptr);

@@ -10,7 +10,9 @@
  * "lvc.autotune" is set, else from the built-in table below.
  *
  * Determinism: a size class always maps to the same choice within a context,
- * so a problem's arithmetic never depends on the rest of its batch.
+ * and a batch runs each problem with its own class's choice (one launch
+ * pair per distinct choice and time step), so a problem's arithmetic never
+ * depends on the rest of its batch.
  *
  * Cache file format: one line per class, "precision numX numY seg_x seg_y".
  */
@@ -67,30 +69,53 @@ static void cache_append(struct futhark_context *ctx, int precision, int nx,
   fclose(f);
 }
 
-/* Built-in choice: the fastest pair measured on MI355X at 256x256 (128
- * options, 63 steps), per call: FP32+refinement x 8 / y 32 (3.60 ms; 8/64
- * 3.86, 8/4 with PCR y 4.22), FP32 8/32 (2.47 ms; 8/8 3.24), FP64 4/32
- * (4.96 ms; 4/4 5.93); also the fastest on the 512-problem uneven batch
- * (FP32+refinement 11.4 ms vs 14.3 for 8/8).  x segments of 8 keep a
- * 256-point row on 32 lanes; the column y sweep (segments 32, 64) factors the
- * shared y matrix once per step.  Columns taller than 8 segments take 64, then
- * fall back to the lane segments of 16. */
-static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
-  (void)nx;
-  (void)ny;
-  if (precision == LVC_FP64) {
-    *sx = 4;
-    *sy = 32;
-  } else if (precision == LVC_FP32) {
-    *sx = 8;
-    *sy = 32;
-  } else {
-    *sx = 8;
-    *sy = 32;
-  }
+/* Built-in choice, measured on MI355X per size class: every candidate pair
+ * timed for numX, numY in {32, 64, 128, 256, 512} in each precision, on two
+ * batches per class (strike variants of one problem, which share their x
+ * tiles, as `main`; and problems that all differ), 47 time steps, 16 to 2048
+ * problems (128 * 256^2 / (numX numY)); the pair below is the one with the
+ * smallest sum of its two times relative to each batch's best.  Most classes
+ * take the general choice (x 8 / y 32; FP64 x 4 / y 32: 4 unknowns per lane
+ * keep its registers in bounds); the exceptions are the classes where it
+ * lost by more than 3%: numX = 32 takes shorter x segments and lane-segment y
+ * solves (the reduced systems are short and the columns few), up to 26%
+ * faster there.  A size maps to the class of the next power of two (32 ..
+ * 512; wider rows take the general choice, fitted). */
+static const struct {
+  int precision, nx, ny, sx, sy;
+} tuned_classes[] = {
+    {LVC_FP32, 32, 32, 4, 4},
+    {LVC_FP32_REFINE, 32, 32, 4, 4},
+    {LVC_FP32_REFINE, 32, 64, 8, 4},
+    {LVC_FP32_REFINE, 32, 128, 8, 4},
+    {LVC_FP32_REFINE, 32, 256, 8, 4},
+    {LVC_FP64, 32, 32, 4, 4},
+    {LVC_FP64, 512, 64, 8, 64},
+};
+
+static int size_class(int n) {
+  int c = 32;
+  while (c < n && c < 1024)
+    c <<= 1;
+  return c;
 }
 
-/* Times one candidate: a batch of 8 problems of the class, 4 time steps. */
+static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
+  const int cx = size_class(nx), cy = size_class(ny);
+  for (size_t i = 0; i < sizeof(tuned_classes) / sizeof(tuned_classes[0]); i++)
+    if (tuned_classes[i].precision == precision && tuned_classes[i].nx == cx &&
+        tuned_classes[i].ny == cy) {
+      *sx = tuned_classes[i].sx;
+      *sy = tuned_classes[i].sy;
+      return;
+    }
+  *sx = precision == LVC_FP64 ? 4 : 8;
+  *sy = 32;
+}
+
+/* Times one candidate: a batch of 8 problems of the class, 4 time steps:
+ * four strike variants of one problem (which share their x tiles, as
+ * `main`'s options) and four problems of their own. */
 static double time_candidate(struct futhark_context *ctx, int precision, int nx,
                              int ny, int sx, int sy) {
   lvc_problem probs[8];
@@ -99,7 +124,7 @@ static double time_candidate(struct futhark_context *ctx, int precision, int nx,
     probs[i].nx = nx;
     probs[i].ny = ny;
     probs[i].nt = 5;
-    probs[i].s0 = 0.03;
+    probs[i].s0 = i < 4 ? 0.03 : 0.03 + 1e-4 * i;
     probs[i].strike = 0.001 * i;
     probs[i].t = 5.0;
     probs[i].alpha = 0.2;

@@ -65,8 +65,9 @@ void futhark_context_config_set_cache_file(struct futhark_context_config *cfg,
  *       (default 0: the cache file, else the built-in per-size table);
  *   "lvc.xshare"  options per x tile sharing one row factorisation, 1..16
  *       (0 = chosen per batch);
- *   "lvc.graph"  1 = replay a batch's step launches from a HIP graph kept
- *       for batches of the same shape (default 0: direct launches).
+ *   "lvc.graph"  1 = replay a call's step launches from the context's kept
+ *       HIP graph: relaunched as is when a call repeats the launches,
+ *       updated (or rebuilt) otherwise (default 0: direct launches).
  * A size too large for the chosen segment length takes the next one that
  * fits.  Returns non-zero for an unknown name or an unsupported value. */
 int futhark_context_config_set_tuning_param(struct futhark_context_config *cfg,

@@ -81,9 +81,11 @@ int futhark_entry_main_noalloc(struct futhark_context *ctx, float *out,
 /* Batched form: problem p has its own sizes (numX[p], numY[p], numT[p]) and
  * parameters and runs LocVolCalib.fut:211-236 (`value`).  All arrays have
  * the same length n.  `precision` as "lvc.precision".  The problems advance
- * together, one launch of each sweep per time step over the problems still
- * running (replayed from a HIP graph with "lvc.graph"), until each has done
- * its own numT-1 steps.  A problem whose price index lies outside its grid
+ * together, per time step one launch of each sweep per segment-pair class
+ * (each problem takes its own size class's pair, so its price is the same
+ * alone and in any batch) over the problems still running (replayed from the
+ * context's kept HIP graph with "lvc.graph"), until each has done its own
+ * numT-1 steps.  A problem whose price index lies outside its grid
  * gives NaN (the call succeeds).  No counterpart in the reference (its
  * `main` shares one size and parameter set). */
 int futhark_entry_main_batch(struct futhark_context *ctx,
