@@ -376,21 +376,23 @@ def bench_serving(lvc, rate, duration_s=1.5, seed=7, max_slots=1024, budget=64 <
             if nxt == n - left:                # nothing outstanding: wait
                 time.sleep(max(0.0, min(arrivals[nxt] - now, 0.001)))
                 continue
-            done = srv.step()
+            srv.step()
             st = srv.stats()
             admitted = st["completed"] - base + st["resident"]
             while adm < admitted:          # admitted by this tick: n_t - 1 ticks from it
                 heapq.heappush(due, (tick + reqs[adm][2] - 2, adm))
                 adm += 1
-            if done > 0:
-                now = time.perf_counter() - t0
-                while due and due[0][0] <= tick:
-                    _, i = heapq.heappop(due)
-                    if srv.poll(ids[i]) is not None:
-                        done_at[i] = now
-                        left -= 1
-                    else:
-                        heapq.heappush(due, (tick + 1, i))
+            # requests due by the previous tick: their prices were copied
+            # behind that tick's launches, which the GPU has run (or is
+            # finishing) while this tick was enqueued
+            now = time.perf_counter() - t0
+            while due and due[0][0] <= tick - 1:
+                _, i = heapq.heappop(due)
+                if srv.poll(ids[i]) is not None:
+                    done_at[i] = now
+                    left -= 1
+                else:
+                    heapq.heappush(due, (tick, i))
             tick += 1
         total = time.perf_counter() - t0
         srv.free()

@@ -148,6 +148,15 @@ int lvc_launch_ysweep(void *stream, int precision, int seg,
 /* The price of each of the nprob problems into out[problem.out_index]. */
 int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
                       int nprob, const void *R, void *out, int out_f64);
+/* The prices of problems list[0 .. nprob-1] into out[0 .. nprob-1]. */
+int lvc_launch_gather_list(void *stream, int precision, lvc_problem *probs,
+                           const int32_t *list, int nprob, const void *R, void *out,
+                           int out_f64);
+/* Sets up problems list[0 .. nprob-1] from the descriptors src[0 .. nprob-1]
+ * (stored to probs[list[i]]). */
+int lvc_launch_setup_list(void *stream, int precision, lvc_problem *probs,
+                          const lvc_problem *src, const int32_t *list, int nprob,
+                          double *geo, void *R);
 
 /* Batched f32 tridiagonal solves of [n][m] systems (the ocean-sim entries):
  * form 0 = the scan form (tridagPar), 1 = Thomas (tridagSeq).  scratch holds

@@ -281,13 +281,17 @@ struct Taps {
 // the descriptor's x0log (the reference's own f32 x[0] for the f32 entry):
 // it decides whether that log is finite, -inf (x[0] = 0) or NaN (x[0] < 0).
 // Grid: (problems, row chunks); chunk 0 also writes the geometry and resets
-// the time index.
+// the time index.  With `list`, block i sets up problem list[i] from the
+// descriptor src[i] (chunk 0 stores it, time index set, to probs[list[i]]).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
+                                                        const lvc_problem *src,
+                                                        const int32_t *list,
                                                         double *geo, T *R) {
 #pragma clang fp contract(off)
-  lvc_problem &pr = probs[blockIdx.x];
+  lvc_problem &dst = probs[list ? list[blockIdx.x] : blockIdx.x];
+  const lvc_problem pr = list ? src[blockIdx.x] : dst;
   const int nx = pr.nx, ny = pr.ny;
   const double s0 = pr.s0, alpha = pr.alpha, nu = pr.nu, beta = pr.beta;
   const double dx = pr.dx, dy = pr.dy;
@@ -315,7 +319,15 @@ __global__ __launch_bounds__(256) void lvc_setup_kernel(lvc_problem *probs,
     x[i] = (double)i * dx - (double)xi * dx + s0;
   for (int j = threadIdx.x; j < ny; j += blockDim.x)
     y[j] = (double)j * dy - (double)yi * dy + logAlpha;
-  if (threadIdx.x == 0) pr.k = pr.nt - 2;
+  if (threadIdx.x == 0) {
+    if (list) {
+      lvc_problem w = pr;
+      w.k = pr.nt - 2;
+      dst = w;
+    } else {
+      dst.k = pr.nt - 2;
+    }
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < LVC_P4(nx); i += blockDim.x) {
     double d0 = 0.0, d2 = 0.0;
@@ -1096,18 +1108,20 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
   }
 }
 
-// value (:236): the price at (yindex, xindex), to the problem's output slot.
+// value (:236): the price at (yindex, xindex), to the problem's output slot
+// (with `list`: of problem list[i], to out[i]).
 template <typename T, typename O>
-__global__ void lvc_gather_kernel(const lvc_problem *probs, int n, const T *R,
-                                  O *out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const lvc_problem &pr = probs[p];
+__global__ void lvc_gather_kernel(const lvc_problem *probs, const int32_t *list, int n,
+                                  const T *R, O *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const lvc_problem &pr = probs[list ? list[i] : i];
+  const int o = list ? i : pr.out_index;
   if (pr.status != 0) {
-    out[pr.out_index] = (O)NAN;
+    out[o] = (O)NAN;
     return;
   }
-  out[pr.out_index] = (O)R[pr.state_off + (int64_t)pr.yindex * pr.nx + pr.xindex];
+  out[o] = (O)R[pr.state_off + (int64_t)pr.yindex * pr.nx + pr.xindex];
 }
 
 // main's descriptors, tiles and factor leaders (lvc_main_layout): thread i
@@ -1261,16 +1275,22 @@ int lvc_seg_supported(int seg, int ysweep) {
   return seg == 4 || seg == 8 || seg == 16 || (ysweep && (seg == 32 || seg == 64));
 }
 
-int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
-                     double *geo, void *R) {
+int lvc_launch_setup_list(void *stream, int precision, lvc_problem *probs,
+                          const lvc_problem *src, const int32_t *list, int nprob,
+                          double *geo, void *R) {
   if (nprob <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(nprob, 8);
   if (precision == LVC_FP64)
-    lvc_setup_kernel<double><<<grid, 256, 0, st>>>(probs, geo, (double *)R);
+    lvc_setup_kernel<double><<<grid, 256, 0, st>>>(probs, src, list, geo, (double *)R);
   else
-    lvc_setup_kernel<float><<<grid, 256, 0, st>>>(probs, geo, (float *)R);
+    lvc_setup_kernel<float><<<grid, 256, 0, st>>>(probs, src, list, geo, (float *)R);
   return (int)hipGetLastError();
+}
+
+int lvc_launch_setup(void *stream, int precision, lvc_problem *probs, int nprob,
+                     double *geo, void *R) {
+  return lvc_launch_setup_list(stream, precision, probs, nullptr, nullptr, nprob, geo, R);
 }
 
 int lvc_launch_xsweep(void *stream, int precision, int seg,
@@ -1291,23 +1311,29 @@ int lvc_launch_ysweep(void *stream, int precision, int seg,
                   false);
 }
 
-int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
-                      int nprob, const void *R, void *out, int out_f64) {
+int lvc_launch_gather_list(void *stream, int precision, lvc_problem *probs,
+                           const int32_t *list, int nprob, const void *R, void *out,
+                           int out_f64) {
   if (nprob <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const int nb = (nprob + 255) / 256;
   if (precision == LVC_FP64) {
     if (out_f64)
-      lvc_gather_kernel<double, double><<<nb, 256, 0, st>>>(probs, nprob, (const double *)R, (double *)out);
+      lvc_gather_kernel<double, double><<<nb, 256, 0, st>>>(probs, list, nprob, (const double *)R, (double *)out);
     else
-      lvc_gather_kernel<double, float><<<nb, 256, 0, st>>>(probs, nprob, (const double *)R, (float *)out);
+      lvc_gather_kernel<double, float><<<nb, 256, 0, st>>>(probs, list, nprob, (const double *)R, (float *)out);
   } else {
     if (out_f64)
-      lvc_gather_kernel<float, double><<<nb, 256, 0, st>>>(probs, nprob, (const float *)R, (double *)out);
+      lvc_gather_kernel<float, double><<<nb, 256, 0, st>>>(probs, list, nprob, (const float *)R, (double *)out);
     else
-      lvc_gather_kernel<float, float><<<nb, 256, 0, st>>>(probs, nprob, (const float *)R, (float *)out);
+      lvc_gather_kernel<float, float><<<nb, 256, 0, st>>>(probs, list, nprob, (const float *)R, (float *)out);
   }
   return (int)hipGetLastError();
+}
+
+int lvc_launch_gather(void *stream, int precision, lvc_problem *probs,
+                      int nprob, const void *R, void *out, int out_f64) {
+  return lvc_launch_gather_list(stream, precision, probs, nullptr, nprob, R, out, out_f64);
 }
 
 }  // extern "C"

@@ -48,9 +48,21 @@ typedef struct {
 } queued_req;
 
 typedef struct {
-  int state; /* 0 queued, 1 resident, 2 done */
+  int state; /* 0 queued, 1 resident, 3 finishing (price copy in flight), 2 done */
   double price;
+  int64_t batch; /* finishing: sequence number of its completion batch */
 } result_rec;
+
+/* Completions of one tick: their prices are copied to pinned host memory
+ * behind the tick's launches and published when the copy has landed. */
+#define LVC_SRV_PENDING 4
+typedef struct {
+  hipEvent_t done;
+  double *host;  /* pinned, max_slots prices */
+  int64_t *ids;  /* request ids, in gather order */
+  int count;
+  int64_t seq;
+} completion_batch;
 
 struct lvc_server {
   struct futhark_context *ctx;
@@ -75,7 +87,15 @@ struct lvc_server {
   int32_t *h_fprobs, *d_fprobs; /* resident slots (factor workgroups) */
   int nfac;
   double *d_out;
-  double *h_out;
+  /* admission: the tick's new descriptors and their slots, staged on the
+   * device for one setup launch */
+  lvc_problem *h_adm, *d_adm;
+  int32_t *h_admslot, *d_admslot;
+  int nadm;
+  /* completions: the tick's finished slots, one gather launch */
+  int32_t *h_done, *d_done;
+  completion_batch pend[LVC_SRV_PENDING];
+  int64_t pend_head, pend_tail; /* batches pend_head .. pend_tail-1 in flight */
   /* tiles of the resident set */
   lvc_tile *h_tiles, *d_tiles;
   int nxt, nyt, tiles_cap, tiles_dirty, ylds;
@@ -100,9 +120,10 @@ static int64_t request_bytes(const struct lvc_server *s, int nx, int ny) {
 /* device bytes of the per-slot tables: descriptor, geometry, y factor
  * table, output, factor list and tiles */
 static int64_t slot_bytes(const struct lvc_server *s, int col) {
-  return (int64_t)sizeof(lvc_problem) + s->slot_geo * (int64_t)sizeof(double) +
+  return 2 * (int64_t)sizeof(lvc_problem) + s->slot_geo * (int64_t)sizeof(double) +
          (col ? LVC_YFAC_LEN * (int64_t)s->real + (int64_t)sizeof(int32_t) : 0) +
-         (int64_t)sizeof(double) + SLOT_MAX_TILES * (int64_t)sizeof(lvc_tile);
+         (int64_t)sizeof(double) + SLOT_MAX_TILES * (int64_t)sizeof(lvc_tile) +
+         2 * (int64_t)sizeof(int32_t);
 }
 
 /* First-fit extent of `len` elements; -1 when none is free. */
@@ -195,23 +216,37 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
   s->h_slots = calloc((size_t)max_slots, sizeof(lvc_problem));
   s->slot_id = malloc((size_t)max_slots * sizeof(int64_t));
   s->steps_left = calloc((size_t)max_slots, sizeof(int));
-  s->h_out = calloc((size_t)max_slots, sizeof(double));
   s->h_fprobs = calloc((size_t)max_slots, sizeof(int32_t));
   s->slot_len = calloc((size_t)max_slots, sizeof(int64_t));
   s->free_ext = malloc(((size_t)max_slots + 1) * sizeof(extent));
   s->tiles_cap = max_slots * SLOT_MAX_TILES;
   s->h_tiles = malloc((size_t)s->tiles_cap * sizeof(lvc_tile));
-  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_out || !s->h_fprobs ||
-           !s->slot_len || !s->free_ext || !s->h_tiles;
-  if (rc)
-    lvc_set_error(ctx, "host allocation failed");
-  s->free_ext[0].off = 0;
-  s->free_ext[0].len = arena;
-  s->nfree = 1;
-  for (int i = 0; !rc && i < max_slots; i++) {
-    s->slot_id[i] = -1;
-    s->h_slots[i].k = -1;
+  s->h_adm = malloc((size_t)max_slots * sizeof(lvc_problem));
+  s->h_admslot = malloc((size_t)max_slots * sizeof(int32_t));
+  s->h_done = malloc((size_t)max_slots * sizeof(int32_t));
+  int rc = !s->h_slots || !s->slot_id || !s->steps_left || !s->h_fprobs || !s->slot_len ||
+           !s->free_ext || !s->h_tiles || !s->h_adm || !s->h_admslot || !s->h_done;
+  for (int b = 0; !rc && b < LVC_SRV_PENDING; b++) {
+    s->pend[b].ids = malloc((size_t)max_slots * sizeof(int64_t));
+    rc = !s->pend[b].ids;
   }
+  if (rc) {
+    lvc_set_error(ctx, "host allocation failed");
+  } else {
+    s->free_ext[0].off = 0;
+    s->free_ext[0].len = arena;
+    s->nfree = 1;
+    for (int i = 0; i < max_slots; i++) {
+      s->slot_id[i] = -1;
+      s->h_slots[i].k = -1;
+    }
+  }
+  for (int b = 0; !rc && b < LVC_SRV_PENDING; b++)
+    rc = lvc_check_hip(ctx, hipHostMalloc((void **)&s->pend[b].host,
+                                          (size_t)max_slots * sizeof(double),
+                                          hipHostMallocDefault), "hipHostMalloc") ||
+         lvc_check_hip(ctx, hipEventCreateWithFlags(&s->pend[b].done, hipEventDisableTiming),
+                       "hipEventCreate");
   if (!rc)
     rc = lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(lvc_problem), (void **)&s->d_slots) ||
          lvc_dev_alloc(ctx, (size_t)max_slots * s->slot_geo * sizeof(double), (void **)&s->d_geo) ||
@@ -219,6 +254,9 @@ struct lvc_server *lvc_server_new(struct futhark_context *ctx, int32_t max_slots
          lvc_dev_alloc(ctx, (size_t)arena * s->real, &s->d_Y) ||
          lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(double), (void **)&s->d_out) ||
          lvc_dev_alloc(ctx, (size_t)s->tiles_cap * sizeof(lvc_tile), (void **)&s->d_tiles) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(lvc_problem), (void **)&s->d_adm) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(int32_t), (void **)&s->d_admslot) ||
+         lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(int32_t), (void **)&s->d_done) ||
          (lvc_seg_is_col(s->seg_y) &&
           (lvc_dev_alloc(ctx, (size_t)max_slots * LVC_YFAC_LEN * s->real, &s->d_yfac) ||
            lvc_dev_alloc(ctx, (size_t)max_slots * sizeof(int32_t), (void **)&s->d_fprobs)));
@@ -249,11 +287,23 @@ void lvc_server_free(struct lvc_server *s) {
     lvc_dev_free(ctx, s->d_fprobs, (size_t)s->max_slots * sizeof(int32_t));
   if (s->d_tiles)
     lvc_dev_free(ctx, s->d_tiles, (size_t)s->tiles_cap * sizeof(lvc_tile));
+  lvc_dev_free(ctx, s->d_adm, (size_t)s->max_slots * sizeof(lvc_problem));
+  lvc_dev_free(ctx, s->d_admslot, (size_t)s->max_slots * sizeof(int32_t));
+  lvc_dev_free(ctx, s->d_done, (size_t)s->max_slots * sizeof(int32_t));
+  for (int b = 0; b < LVC_SRV_PENDING; b++) {
+    if (s->pend[b].host)
+      (void)hipHostFree(s->pend[b].host);
+    if (s->pend[b].done)
+      (void)hipEventDestroy(s->pend[b].done);
+    free(s->pend[b].ids);
+  }
   pthread_mutex_unlock(&ctx->lock);
   free(s->h_slots);
   free(s->slot_id);
   free(s->steps_left);
-  free(s->h_out);
+  free(s->h_adm);
+  free(s->h_admslot);
+  free(s->h_done);
   free(s->h_fprobs);
   free(s->slot_len);
   free(s->free_ext);
@@ -409,12 +459,50 @@ static int admit(struct lvc_server *s) {
     s->nresident++;
     s->qhead++;
     s->tiles_dirty = 1;
-    if (lvc_upload(ctx, s->d_slots + slot, &q, sizeof(q)))
+    s->h_adm[s->nadm] = q;
+    s->h_admslot[s->nadm] = slot;
+    s->nadm++;
+  }
+  /* the tick's admissions: their descriptors and slots in one upload each,
+   * set up by one launch (which also stores each descriptor to its slot) */
+  if (s->nadm > 0) {
+    const int n = s->nadm;
+    s->nadm = 0;
+    if (lvc_upload(ctx, s->d_adm, s->h_adm, (size_t)n * sizeof(lvc_problem)) ||
+        lvc_upload(ctx, s->d_admslot, s->h_admslot, (size_t)n * sizeof(int32_t)))
       return 1;
-    int e = lvc_launch_setup((void *)ctx->stream, s->precision, s->d_slots + slot,
-                             1, s->d_geo, s->d_R);
+    const int e = lvc_launch_setup_list((void *)ctx->stream, s->precision, s->d_slots, s->d_adm,
+                                        s->d_admslot, n, s->d_geo, s->d_R);
     if (lvc_check_hip(ctx, e, "setup kernel"))
       return 1;
+  }
+  return 0;
+}
+
+/* Publishes the completion batches whose price copies have landed, in
+ * order; with `upto` >= 0, first waits for every batch up to that sequence
+ * number. */
+static int publish(struct lvc_server *s, int64_t upto) {
+  struct futhark_context *ctx = s->ctx;
+  while (s->pend_head < s->pend_tail) {
+    completion_batch *b = &s->pend[s->pend_head % LVC_SRV_PENDING];
+    if (b->seq > upto) {
+      const hipError_t q = hipEventQuery(b->done);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError(); /* not an error: the copy is still in flight */
+        return 0;
+      }
+      if (lvc_check_hip(ctx, q, "hipEventQuery"))
+        return 1;
+    } else if (lvc_check_hip(ctx, hipEventSynchronize(b->done), "hipEventSynchronize")) {
+      return 1;
+    }
+    for (int j = 0; j < b->count; j++) {
+      result_rec *r = &s->results[b->ids[j]];
+      r->price = b->host[j];
+      r->state = 2;
+    }
+    s->pend_head++;
   }
   return 0;
 }
@@ -424,7 +512,9 @@ int32_t lvc_server_step(struct lvc_server *s) {
     return -1;
   struct futhark_context *ctx = s->ctx;
   pthread_mutex_lock(&ctx->lock);
-  int rc = admit(s);
+  int rc = publish(s, -1);
+  if (!rc)
+    rc = admit(s);
   if (!rc && s->tiles_dirty)
     rc = rebuild_tiles(s);
   int32_t ndone = 0;
@@ -445,29 +535,44 @@ int32_t lvc_server_step(struct lvc_server *s) {
       lvc_prof_end(ctx, tok);
       rc = lvc_check_hip(ctx, e, "y sweep kernel");
     }
-    /* completions */
+    /* completions: one gather over the finished slots, its prices copied
+     * to a pinned buffer behind the launches and published once landed
+     * (poll waits for them); the slots and arena extents are free at once,
+     * since every later use of them runs after the gather in stream order */
     for (int i = 0; !rc && i < s->max_slots; i++) {
       if (s->slot_id[i] < 0)
         continue;
       if (--s->steps_left[i] > 0)
         continue;
-      e = lvc_launch_gather(st, s->precision, s->d_slots + i, 1, s->d_R,
-                            s->d_out + i, 1);
-      rc = lvc_check_hip(ctx, e, "gather kernel");
-      ndone++;
+      s->h_done[ndone++] = i;
     }
+    completion_batch *cb = NULL;
     if (!rc && ndone > 0) {
-      rc = lvc_check_hip(ctx, hipMemcpyAsync(s->h_out, s->d_out,
-                                             (size_t)s->max_slots * sizeof(double),
-                                             hipMemcpyDeviceToHost, ctx->stream),
-                         "hipMemcpyAsync") ||
-           lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-      for (int i = 0; !rc && i < s->max_slots; i++) {
-        if (s->slot_id[i] < 0 || s->steps_left[i] > 0)
-          continue;
+      if (s->pend_tail - s->pend_head == LVC_SRV_PENDING)
+        rc = publish(s, s->pend_head);            /* the ring is full: the oldest */
+      cb = &s->pend[s->pend_tail % LVC_SRV_PENDING];
+      if (!rc)
+        rc = lvc_upload(ctx, s->d_done, s->h_done, (size_t)ndone * sizeof(int32_t));
+      if (!rc)
+        rc = lvc_check_hip(ctx, lvc_launch_gather_list(st, s->precision, s->d_slots, s->d_done,
+                                                       ndone, s->d_R, s->d_out, 1),
+                           "gather kernel") ||
+             lvc_check_hip(ctx, hipMemcpyAsync(cb->host, s->d_out, (size_t)ndone * sizeof(double),
+                                               hipMemcpyDeviceToHost, ctx->stream),
+                           "hipMemcpyAsync") ||
+             lvc_check_hip(ctx, hipEventRecord(cb->done, ctx->stream), "hipEventRecord");
+      if (!rc) {
+        cb->count = ndone;
+        cb->seq = s->pend_tail++;
+      }
+    }
+    for (int j = 0; !rc && j < ndone; j++) {
+      const int i = s->h_done[j];
+      {
         const int64_t id = s->slot_id[i];
-        s->results[id].state = 2;
-        s->results[id].price = s->h_out[i];
+        cb->ids[j] = id;
+        s->results[id].state = 3;
+        s->results[id].batch = cb->seq;
         s->resident_bytes -= request_bytes(s, s->h_slots[i].nx, s->h_slots[i].ny);
         arena_give(s, s->h_slots[i].state_off, s->slot_len[i]);
         s->slot_len[i] = 0;
@@ -486,10 +591,19 @@ int32_t lvc_server_step(struct lvc_server *s) {
 int32_t lvc_server_poll(struct lvc_server *s, int64_t id, double *price) {
   if (!s || id < 0 || id >= s->nresults)
     return -1;
-  if (s->results[id].state != 2)
+  struct futhark_context *ctx = s->ctx;
+  pthread_mutex_lock(&ctx->lock);
+  /* finished on the device: wait for its price copy (a tick at most) */
+  int rc = s->results[id].state == 3 ? publish(s, s->results[id].batch) : publish(s, -1);
+  const int done = s->results[id].state == 2;
+  const double v = s->results[id].price;
+  pthread_mutex_unlock(&ctx->lock);
+  if (rc)
+    return -1;
+  if (!done)
     return 0;
   if (price)
-    *price = s->results[id].price;
+    *price = v;
   return 1;
 }
 
@@ -509,7 +623,10 @@ int32_t lvc_server_drain(struct lvc_server *s) {
       return -1;
     }
   }
-  return 0;
+  pthread_mutex_lock(&s->ctx->lock);
+  const int rc = publish(s, s->pend_tail - 1);
+  pthread_mutex_unlock(&s->ctx->lock);
+  return rc ? -1 : 0;
 }
 
 int64_t lvc_server_device_bytes(struct lvc_server *s) {

@@ -69,46 +69,21 @@ static void cache_append(struct futhark_context *ctx, int precision, int nx,
   fclose(f);
 }
 
-/* Built-in choice, measured on MI355X per size class: every candidate pair
- * timed for numX, numY in {32, 64, 128, 256, 512} in each precision, on two
- * batches per class (strike variants of one problem, which share their x
- * tiles, as `main`; and problems that all differ), 47 time steps, 16 to 2048
- * problems (128 * 256^2 / (numX numY)); the pair below is the one with the
- * smallest sum of its two times relative to each batch's best.  Most classes
- * take the general choice (x 8 / y 32; FP64 x 4 / y 32: 4 unknowns per lane
- * keep its registers in bounds); the exceptions are the classes where it
- * lost by more than 3%: numX = 32 takes shorter x segments and lane-segment y
- * solves (the reduced systems are short and the columns few), up to 26%
- * faster there.  A size maps to the class of the next power of two (32 ..
- * 512; wider rows take the general choice, fitted). */
-static const struct {
-  int precision, nx, ny, sx, sy;
-} tuned_classes[] = {
-    {LVC_FP32, 32, 32, 4, 4},
-    {LVC_FP32_REFINE, 32, 32, 4, 4},
-    {LVC_FP32_REFINE, 32, 64, 8, 4},
-    {LVC_FP32_REFINE, 32, 128, 8, 4},
-    {LVC_FP32_REFINE, 32, 256, 8, 4},
-    {LVC_FP64, 32, 32, 4, 4},
-    {LVC_FP64, 512, 64, 8, 64},
-};
-
-static int size_class(int n) {
-  int c = 32;
-  while (c < n && c < 1024)
-    c <<= 1;
-  return c;
-}
-
+/* Built-in choice: x 8 / y 32 (FP64 x 4 / y 32: 4 unknowns per lane keep
+ * its registers in bounds), fitted to the size.  Measured on MI355X for
+ * every class numX, numY in {32, 64, 128, 256, 512} in each precision, on a
+ * batch of strike variants (shared x tiles, as `main`) and a batch of
+ * distinct problems, 47 steps, 16-2048 problems (128 * 256^2 / (numX
+ * numY)): it was the fastest pair, or within 3% of it, in 68 of the 75
+ * classes.  The other seven (numX = 32 with lane-segment y solves of 4, up
+ * to 26% faster there, and FP64 512 x 64 with y 64) won only on such large
+ * batches of one class: on main(16, 32, 256, 256) x 8 / y 4 took 7.27 ms
+ * against 4.87, and the 512-problem uneven batch 19.6 ms against 10.5
+ * (split into three segment classes).  So no class departs from it; the
+ * autotuner ("lvc.autotune") times a class on a small batch of its own. */
 static void builtin(int precision, int nx, int ny, int *sx, int *sy) {
-  const int cx = size_class(nx), cy = size_class(ny);
-  for (size_t i = 0; i < sizeof(tuned_classes) / sizeof(tuned_classes[0]); i++)
-    if (tuned_classes[i].precision == precision && tuned_classes[i].nx == cx &&
-        tuned_classes[i].ny == cy) {
-      *sx = tuned_classes[i].sx;
-      *sy = tuned_classes[i].sy;
-      return;
-    }
+  (void)nx;
+  (void)ny;
   *sx = precision == LVC_FP64 ? 4 : 8;
   *sy = 32;
 }

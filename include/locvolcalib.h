@@ -129,11 +129,14 @@ void lvc_server_free(struct lvc_server *srv);
  * request that can never fit the memory budget; see get_error). */
 int64_t lvc_server_submit(struct lvc_server *srv, const struct lvc_request *req);
 /* Runs one engine tick: admits queued requests, advances every resident
- * request by one time step, completes the ones that finished.  Returns the
- * number of requests completed by this tick, or -1 on error. */
+ * request by one time step, completes the ones that finished.  Enqueues its
+ * work on the context's stream without waiting for it: the prices of the
+ * requests a tick completes are copied back behind its launches.  Returns
+ * the number of requests completed by this tick, or -1 on error. */
 int32_t lvc_server_step(struct lvc_server *srv);
 /* Completed result for `id`: returns 1 and stores the price, 0 when not yet
- * complete, -1 for an unknown id. */
+ * complete, -1 for an unknown id.  For a request a tick has completed, waits
+ * for that tick's price copy if it has not landed yet. */
 int32_t lvc_server_poll(struct lvc_server *srv, int64_t id, double *price);
 /* Ticks until nothing is queued or resident.  Returns 0 or -1. */
 int32_t lvc_server_drain(struct lvc_server *srv);

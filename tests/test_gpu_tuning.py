@@ -3,8 +3,9 @@ version selection at LocVolCalib.fut:249) on the GPU, through the C ABI:
 
   * a problem runs with its own size class's segment pair, so its prices are
     the same bits alone and inside a batch of other sizes;
-  * the built-in per-size table is what an unset configuration uses (checked
-    by bit equality with the pair forced through the tuning parameters);
+  * the built-in choice, fitted to each size, is what an unset configuration
+    uses (checked by bit equality with the pair forced through the tuning
+    parameters);
   * "lvc.autotune" times the candidates, writes the choice to the cache file,
     and a new context reads it back (same bits as the pair forced);
   * graph replay: the kept graph is launched again for a repeated call and
@@ -44,9 +45,10 @@ def take(pr, idx):
     return {k: np.asarray(v)[idx] for k, v in pr.items()}
 
 
-# classes with their own entry in the built-in table (32-wide rows) beside
-# classes with the general choice
-SHAPES = [(32, 32), (32, 64), (64, 48), (256, 256), (32, 128), (128, 32), (512, 64), (40, 30)]
+# sizes whose fitted segment pairs differ (rows wider than 512 take x 16, in
+# FP64 rows of 512 take x 8; columns taller than 256 take y 64), so that a
+# mixed batch runs as several sub-batches
+SHAPES = [(32, 32), (600, 40), (64, 48), (256, 256), (48, 300), (128, 32), (512, 64), (40, 30)]
 
 
 @pytest.mark.parametrize("precision", PRECS)
@@ -68,11 +70,13 @@ def test_problem_bits_do_not_depend_on_the_batch(precision):
     assert np.max(np.abs(mixed - ref)) / np.max(np.abs(ref)) < bar
 
 
-# (precision, numX, numY) -> the built-in pair (measured per size class)
-BUILTIN = [(lvc.PRECISION_FP32_REFINE, 32, 32, 4, 4), (lvc.PRECISION_FP32_REFINE, 32, 64, 8, 4),
-           (lvc.PRECISION_FP32_REFINE, 256, 256, 8, 32), (lvc.PRECISION_FP64, 32, 32, 4, 4),
-           (lvc.PRECISION_FP64, 512, 64, 8, 64), (lvc.PRECISION_FP64, 128, 128, 4, 32),
-           (lvc.PRECISION_FP32, 32, 32, 4, 4), (lvc.PRECISION_FP32, 64, 64, 8, 32)]
+# (precision, numX, numY) -> the built-in pair: x 8 / y 32 (FP64 x 4), fitted
+# to the size (rows of 512 need 8 per lane for <= 64 lanes; columns taller
+# than 8 segments of 32 take 64, then lane segments of 16)
+BUILTIN = [(lvc.PRECISION_FP32_REFINE, 32, 32, 8, 32), (lvc.PRECISION_FP32_REFINE, 256, 256, 8, 32),
+           (lvc.PRECISION_FP32_REFINE, 1024, 300, 16, 64), (lvc.PRECISION_FP64, 128, 128, 4, 32),
+           (lvc.PRECISION_FP64, 512, 64, 8, 32), (lvc.PRECISION_FP64, 64, 520, 4, 16),
+           (lvc.PRECISION_FP32, 32, 32, 8, 32), (lvc.PRECISION_FP32, 64, 512, 8, 64)]
 
 
 @pytest.mark.parametrize("precision,nx,ny,sx,sy", BUILTIN)
