@@ -31,7 +31,9 @@ PRECISION_FP32_REFINE = 1
 PRECISION_FP64 = 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-library_path = os.path.join(_HERE, "liblocvolcalib.so")
+# LVC_LIBRARY names another build of the same sources (the tests' sanitizer
+# build of the host code, _build/asan/liblocvolcalib.so)
+library_path = os.environ.get("LVC_LIBRARY") or os.path.join(_HERE, "liblocvolcalib.so")
 _lib: Optional[ctypes.CDLL] = None
 
 _vp = ctypes.c_void_p

@@ -51,7 +51,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liblvc_oracle.so")
+# LVC_ORACLE_LIB names another build of the same sources (the tests' sanitizer
+# build, oracle/_build/liblvc_oracle_asan.so)
+_LIB_PATH = os.environ.get("LVC_ORACLE_LIB") or os.path.join(_HERE, "_build", "liblvc_oracle.so")
 _lib = None
 
 _c_i32 = ctypes.c_int32
