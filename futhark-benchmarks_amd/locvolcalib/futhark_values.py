@@ -159,3 +159,25 @@ def dumps(values: Iterable[Value]) -> str:
                 return "[" + ", ".join(rec(e) for e in x) + "]"
             out.append(rec(a))
     return "\n".join(out) + "\n"
+
+
+def dumps_binary(values: Iterable[Value]) -> bytes:
+    """Write values in the Futhark binary format: per value 'b', version 2,
+    the rank, the 4-character type name, the dimensions as little-endian
+    u64 and the elements little-endian (the layout pbbs/pbbs2fut.c:13-23
+    writes and _parse_binary reads)."""
+    names = {np.dtype(t): k for k, t in _BIN_TYPES.items()}
+    out = bytearray()
+    for v in values:
+        a = np.asarray(v)
+        if a.dtype not in names:
+            raise ValueError(f"no Futhark binary type for {a.dtype}")
+        out += b"b" + bytes([2, a.ndim]) + names[a.dtype]
+        out += np.asarray(a.shape, dtype="<u8").tobytes()
+        out += np.ascontiguousarray(a, dtype=a.dtype.newbyteorder("<")).tobytes()
+    return bytes(out)
+
+
+def dump_binary(values: Iterable[Value], path: str) -> None:
+    with open(path, "wb") as f:
+        f.write(dumps_binary(values))
