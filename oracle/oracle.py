@@ -377,3 +377,179 @@ def op_payoffs(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, m
                     int(nthreads or os.cpu_count() or 1)) != 0:
         raise ValueError(f"contract {contract} does not take [{D}][{U}] paths")
     return out
+
+
+# ---- InterestCalib (finpar/InterestCalib.fut), oracle/ic_oracle.c ----------
+# The G2++ swaption pricer, the Cauchy log-likelihood and the genetic / DE
+# MCMC calibration loop in f32 in the reference's order of operations; the
+# calendar of lib/github.com/diku-dk/date.  Pinned in tests/test_ic_oracle.py.
+
+def _ic_sig(name, res, args):
+    f = getattr(lib(), name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+def ic_set_mutation(m: int) -> None:
+    """Plant mistake m (0 = none) in the InterestCalib oracle: 1 the Gray
+    code dropped from sobolInd, 2 erf's a3 coefficient negated, 3 the sign of
+    the cross term t3 of bigv, 4 rhoxy without its b_fun(a+b) factor, 5 the
+    last coupon without its notional, 6 add_months without the end-of-month
+    clamp."""
+    _ic_sig("ic_oracle_set_mutation", None, [ctypes.c_int])(int(m))
+
+
+class ic_mutation:
+    """Context manager planting an InterestCalib oracle mistake."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        ic_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        ic_set_mutation(0)
+
+
+def ic_date_of_triple(y: int, m: int, d: int) -> int:
+    return int(_ic_sig("ic_date_of_triple", _c_i32, [_c_i32] * 3)(y, m, d))
+
+
+def ic_triple_of_date(t: int) -> tuple:
+    y, m, d = _c_i32(), _c_i32(), _c_i32()
+    _ic_sig("ic_triple_of_date", None, [_c_i32] + [ctypes.POINTER(_c_i32)] * 3)(
+        t, ctypes.byref(y), ctypes.byref(m), ctypes.byref(d))
+    return (y.value, m.value, d.value)
+
+
+def ic_check_date(y: int, m: int, d: int) -> bool:
+    return bool(_ic_sig("ic_check_date", ctypes.c_int, [_c_i32] * 3)(y, m, d))
+
+
+def ic_add_months(t: int, months: int) -> int:
+    return int(_ic_sig("ic_add_months", _c_i32, [_c_i32, _c_i32])(t, months))
+
+
+def ic_diff_dates(t1: int, t2: int) -> float:
+    return float(_ic_sig("ic_diff_dates", ctypes.c_double, [_c_i32, _c_i32])(t1, t2))
+
+
+def ic_ugaussian_P(x: float) -> float:
+    f = _ic_sig("ic_ugaussian_P", ctypes.c_float, [ctypes.c_float])
+    return float(f(ctypes.c_float(x)))
+
+
+def ic_ugaussian_P_exp(x: float, exp_factor: float) -> float:
+    f = _ic_sig("ic_ugaussian_P_exp", ctypes.c_float, [ctypes.c_float, ctypes.c_float])
+    return float(f(ctypes.c_float(x), ctypes.c_float(exp_factor)))
+
+
+def ic_sobol(n: int, dir_vct: np.ndarray) -> float:
+    """sobolInd (:190-202) of index n over the direction vector."""
+    dv = np.ascontiguousarray(dir_vct, dtype=np.int32)
+    f = _ic_sig("ic_sobol_ind", ctypes.c_float, [ctypes.c_int, _ptr, _c_i32])
+    return float(f(len(dv), _p(dv), int(n)))
+
+
+def ic_brent(scales, bbi, lb: float, ub: float, tol=1.0e-4, iter_max=1000) -> tuple:
+    """rootFinding_Brent (:124-185) on sum(scales * exp(-bbi * y)) - 1:
+    (root, error); error is +-inf where [lb, ub] does not bracket a root."""
+    sc = np.ascontiguousarray(scales, np.float32)
+    bb = np.ascontiguousarray(bbi, np.float32)
+    err = ctypes.c_float()
+    f = _ic_sig("ic_brent", ctypes.c_float, [ctypes.c_int, _ptr, _ptr, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_float, _c_i32,
+                                             ctypes.POINTER(ctypes.c_float)])
+    r = f(len(sc), _p(sc), _p(bb), lb, ub, tol, iter_max, ctypes.byref(err))
+    return float(r), float(err.value)
+
+
+def ic_eval(genome, swaptions, herm_x, herm_w) -> tuple:
+    """evalGenomeOnSwap (:484-602) of one genome (a, b, rho, nu, sigma) on
+    every swaption [n][4]: (quotes, prices), float32 [n] each."""
+    g = np.ascontiguousarray(genome, np.float32)
+    sw = np.ascontiguousarray(swaptions, np.float32).reshape(-1, 4)
+    hx = np.ascontiguousarray(herm_x, np.float32)
+    hw = np.ascontiguousarray(herm_w, np.float32)
+    f = _ic_sig("ic_eval_genome_on_swap", None,
+                [_ptr, ctypes.c_int, _ptr, _ptr, _ptr, ctypes.POINTER(ctypes.c_float),
+                 ctypes.POINTER(ctypes.c_float)])
+    q, p = ctypes.c_float(), ctypes.c_float()
+    quotes = np.empty(len(sw), np.float32)
+    prices = np.empty(len(sw), np.float32)
+    for s in range(len(sw)):
+        row = np.ascontiguousarray(sw[s])
+        f(_p(g), len(hx), _p(hx), _p(hw), _p(row), ctypes.byref(q), ctypes.byref(p))
+        quotes[s], prices[s] = q.value, p.value
+    return quotes, prices
+
+
+def ic_log_likelihood(y_ref: float, y: float) -> float:
+    f = _ic_sig("ic_log_likelihood", ctypes.c_float, [ctypes.c_float, ctypes.c_float])
+    return float(f(y_ref, y))
+
+
+def ic_logliks(genomes, swaptions, herm_x, herm_w, nthreads=None) -> np.ndarray:
+    """The log-likelihood (:639-645, a sequential f32 sum over the swaptions)
+    of each genome [n][5]."""
+    g = np.ascontiguousarray(genomes, np.float32).reshape(-1, 5)
+    sw = np.ascontiguousarray(swaptions, np.float32).reshape(-1, 4)
+    hx = np.ascontiguousarray(herm_x, np.float32)
+    hw = np.ascontiguousarray(herm_w, np.float32)
+    out = np.empty(len(g), np.float32)
+    _ic_sig("ic_logliks", None, [ctypes.c_int, _ptr, ctypes.c_int, _ptr, ctypes.c_int, _ptr,
+                                 _ptr, _ptr, ctypes.c_int])(
+        len(g), _p(g), len(sw), _p(sw), len(hx), _p(hx), _p(hw), _p(out),
+        int(nthreads or os.cpu_count() or 1))
+    return out
+
+
+def ic_propose(sob_offs: int, genomes, props, dir_vct) -> tuple:
+    """One iteration's proposals (:655-724) from the Sobol offset at the top
+    of the iteration: (new proposals [pop][5], fb ratios [pop], the Sobol
+    offset of the acceptance draws)."""
+    g = np.ascontiguousarray(genomes, np.float32).reshape(-1, 5)
+    pr = np.array(props, np.float32).reshape(-1, 5)
+    dv = np.ascontiguousarray(dir_vct, np.int32)
+    fb = np.empty(len(g), np.float32)
+    acc = _ic_sig("ic_propose", _c_i32, [_c_i32, _c_i32, _ptr, _c_i32, _ptr, _ptr, _ptr])(
+        len(g), len(dv), _p(dv), int(sob_offs), _p(g), _p(pr), _p(fb))
+    return pr, fb, int(acc)
+
+
+def ic_accept(acc_offs: int, genomes, logliks, props, newl, fb, dir_vct) -> tuple:
+    """The accept/reject step (:733-745): (genomes, logliks) after it."""
+    g = np.array(genomes, np.float32).reshape(-1, 5)
+    ll = np.array(logliks, np.float32)
+    pr = np.ascontiguousarray(props, np.float32).reshape(-1, 5)
+    nl = np.ascontiguousarray(newl, np.float32)
+    f_ = np.ascontiguousarray(fb, np.float32)
+    dv = np.ascontiguousarray(dir_vct, np.int32)
+    _ic_sig("ic_accept", None, [_c_i32, _c_i32, _ptr, _c_i32] + [_ptr] * 5)(
+        len(g), len(dv), _p(dv), int(acc_offs), _p(g), _p(ll), _p(pr), _p(nl), _p(f_))
+    return g, ll
+
+
+def ic_main(pop, mcmc_conv, swaptions, herm_x, herm_w, dir_vct, nthreads=None) -> tuple:
+    """main (:929-936): (six f32 scalars a, b, sigma, nu, rho, log-likelihood
+    of the winner, summary [n][3], final genomes [pop][5], final
+    log-likelihoods [pop]).  Raises ValueError where the reference fails."""
+    sw = np.ascontiguousarray(swaptions, np.float32).reshape(-1, 4)
+    hx = np.ascontiguousarray(herm_x, np.float32)
+    hw = np.ascontiguousarray(herm_w, np.float32)
+    dv = np.ascontiguousarray(dir_vct, np.int32)
+    out6 = np.empty(6, np.float32)
+    summ = np.empty((len(sw), 3), np.float32)
+    fg = np.empty((max(int(pop), 0), 5), np.float32)
+    fl = np.empty(max(int(pop), 0), np.float32)
+    rc = _ic_sig("ic_main", ctypes.c_int,
+                 [_c_i32, _c_i32, _c_i32, _ptr, _c_i32, _ptr, _ptr, _c_i32, _ptr, _ptr, _ptr,
+                  _ptr, _ptr, ctypes.c_int])(
+        int(pop), int(mcmc_conv), len(sw), _p(sw), len(hx), _p(hx), _p(hw), len(dv), _p(dv),
+        _p(out6), _p(summ), _p(fg), _p(fl), int(nthreads or os.cpu_count() or 1))
+    if rc != 0:
+        raise ValueError("InterestCalib main: the reference fails on these arguments")
+    return out6, summ, fg, fl

@@ -28,6 +28,22 @@ def load_dataset(name):
     return inp, np.asarray(out)
 
 
+def load_ic_dataset(name):
+    """finpar/InterestCalib-data/<name>.{in,out} (copied verbatim): main's
+    arguments (pop, mcmc_conv, swaptions [n][4], hermCoefs, hermWeights,
+    sobDirVct) and the expected outputs (six f32 scalars, summary [n][3])."""
+    from locvolcalib import futhark_values as fv
+    d = os.path.join(GOLDEN, "InterestCalib-data")
+    v = fv.load(os.path.join(d, f"{name}.in"))
+    o = fv.load(os.path.join(d, f"{name}.out"))
+    args = dict(pop=int(v[0]), mcmc_conv=int(v[1]),
+                swaptions=np.ascontiguousarray(v[3], np.float32),
+                herm_x=np.ascontiguousarray(v[5], np.float32),
+                herm_w=np.ascontiguousarray(v[6], np.float32),
+                dir_vct=np.ascontiguousarray(v[8], np.int32))
+    return args, np.array([float(x) for x in o[:6]], np.float32), np.asarray(o[6], np.float64)
+
+
 # initGrid's scalars in f32 (the reference's f32 entry).
 # (t, alpha) for which 20*alpha*sqrt(t) rounds to a power of two in f32, so
 # that s0/dx = numX/(20*alpha*sqrt(t)) is an integer in f32 (LocVolCalib.fut:
