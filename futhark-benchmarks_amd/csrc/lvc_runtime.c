@@ -606,7 +606,9 @@ DEFINE_RAW(f32_1d, float)
 DEFINE_RAW(f64_1d, double)
 
 /* Internal accessors for the engine. */
+void *lvc_array_mem_f32(const struct futhark_f32_1d *a) { return a->a.mem; }
 void *lvc_array_mem_f64(const struct futhark_f64_1d *a) { return a->a.mem; }
+int64_t lvc_array_len_f32(const struct futhark_f32_1d *a) { return a->a.shape[0]; }
 void *lvc_array_mem_i32(const struct futhark_i32_1d *a) { return a->a.mem; }
 int64_t lvc_array_len_f64(const struct futhark_f64_1d *a) { return a->a.shape[0]; }
 int64_t lvc_array_len_i32(const struct futhark_i32_1d *a) { return a->a.shape[0]; }

@@ -126,7 +126,9 @@ int lvc_upload(struct futhark_context *ctx, void *dst, const void *src,
 
 /* the 1-D arrays' device memory and length; adopt wraps device memory from
  * lvc_dev_alloc as an array that futhark_free_*_1d returns to the pool */
+void *lvc_array_mem_f32(const struct futhark_f32_1d *a);
 void *lvc_array_mem_f64(const struct futhark_f64_1d *a);
+int64_t lvc_array_len_f32(const struct futhark_f32_1d *a);
 void *lvc_array_mem_i32(const struct futhark_i32_1d *a);
 int64_t lvc_array_len_f64(const struct futhark_f64_1d *a);
 int64_t lvc_array_len_i32(const struct futhark_i32_1d *a);

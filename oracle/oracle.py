@@ -533,6 +533,13 @@ def ic_accept(acc_offs: int, genomes, logliks, props, newl, fb, dir_vct) -> tupl
     return g, ll
 
 
+def ic_schedule_has_de(pop: int, mcmc_conv: int, dir_vct) -> bool:
+    """Whether a DE move comes up in the first mcmc_conv iterations."""
+    dv = np.ascontiguousarray(dir_vct, np.int32)
+    return bool(_ic_sig("ic_schedule_has_de", ctypes.c_int, [_c_i32, _c_i32, _c_i32, _ptr])(
+        int(pop), int(mcmc_conv), len(dv), _p(dv)))
+
+
 def ic_main(pop, mcmc_conv, swaptions, herm_x, herm_w, dir_vct, nthreads=None) -> tuple:
     """main (:929-936): (six f32 scalars a, b, sigma, nu, rho, log-likelihood
     of the winner, summary [n][3], final genomes [pop][5], final

@@ -10,12 +10,14 @@ import pytest
 
 from conftest import ROOT
 import locvolcalib as lvc
+from locvolcalib import interestcalib as icl
 from locvolcalib import optionpricing as op
 
 HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 # Each generated library declares the context and array API beside its own
-# entry points; optionpricing.h belongs to liboptionpricing.so.
-OWN_HEADERS = {"optionpricing.h"}
+# entry points; optionpricing.h belongs to liboptionpricing.so and
+# interestcalib.h to libinterestcalib.so.
+OWN_HEADERS = {"optionpricing.h", "interestcalib.h"}
 SHARED = ("futhark_context.h", "futhark_arrays.h")
 
 
@@ -27,7 +29,7 @@ def header_functions(library="locvolcalib"):
         hs = [os.path.join(ROOT, "include", n) for n in SHARED + (library + ".h",)]
     for h in hs:
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
-        names |= set(re.findall(r"\b((?:futhark|lvc)_[A-Za-z0-9_]+)\s*\(", text))
+        names |= set(re.findall(r"\b((?:futhark|lvc|interestcalib)_[A-Za-z0-9_]+)\s*\(", text))
     return sorted(names)
 
 
@@ -183,6 +185,58 @@ def test_optionpricing_without_device_reports_error():
         assert "device" in lvc._take_string(L.futhark_context_get_error(ctx)).lower()
         with pytest.raises(op.OptionPricingError):
             op.OptionPricing()
+    finally:
+        L.futhark_context_free(ctx)
+        L.futhark_context_config_free(cfg)
+
+
+# ---- libinterestcalib.so ---------------------------------------------------
+
+def test_interestcalib_exports_every_declared_function():
+    L = icl.lib()
+    names = header_functions("interestcalib")
+    assert {"futhark_entry_main", "interestcalib_trace", "interestcalib_eval",
+            "futhark_new_f32_2d", "futhark_new_i32_1d"} <= set(names)
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in the header but not exported"
+
+
+def test_interestcalib_binding_covers_header():
+    assert set(header_functions("interestcalib")) == set(icl.exported_functions())
+
+
+def test_interestcalib_only_public_symbols_exported():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", icl.library_path],
+                         capture_output=True, text=True, check=True).stdout
+    syms = [l.split()[-1] for l in out.splitlines() if l.strip()]
+    assert syms and all(s.startswith(("futhark_", "interestcalib_")) for s in syms), syms
+
+
+def test_interestcalib_has_no_tuning_params():
+    L = icl.lib()
+    assert L.futhark_get_tuning_param_count() == 0
+    cfg = L.futhark_context_config_new()
+    assert L.futhark_context_config_set_tuning_param(cfg, b"lvc.precision", 1) != 0
+    L.futhark_context_config_free(cfg)
+
+
+def test_interestcalib_without_device_reports_error():
+    L = icl.lib()
+    cfg = L.futhark_context_config_new()
+    ctx = L.futhark_context_new(cfg)
+    try:
+        err = lvc._take_string(L.futhark_context_get_error(ctx))
+        if err is None:
+            pytest.skip("a GPU is present; this checks the no-device path")
+        outs = [ctypes.c_float() for _ in range(6)]
+        arr = ctypes.c_void_p()
+        rc = L.futhark_entry_main(ctx, *[ctypes.byref(o) for o in outs], ctypes.byref(arr), 8,
+                                  4, 0, None, 0, None, None, 0, None)
+        assert rc != 0 and not arr.value
+        assert "device" in lvc._take_string(L.futhark_context_get_error(ctx)).lower()
+        with pytest.raises(icl.InterestCalibError):
+            icl.InterestCalib()
     finally:
         L.futhark_context_free(ctx)
         L.futhark_context_config_free(cfg)
