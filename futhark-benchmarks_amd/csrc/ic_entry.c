@@ -4,9 +4,10 @@
  * sizes impose, fixes the move schedule (which depends only on Sobol
  * numbers at data-independent offsets), lays out the swaption tables and
  * the wavefront work lists, and launches csrc/ic_kernels.hip on the
- * context's stream: per call one setup, one initial evaluation, then two
- * launches per MCMC iteration, the winner and its summary, with one
- * synchronisation at the end for the six scalar results.
+ * context's stream: per call one setup, one initial evaluation, then three
+ * launches per MCMC iteration (evaluate, accept, propose), the winner and
+ * its summary, with one synchronisation at the end for the six scalar
+ * results.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -61,10 +62,18 @@ static void move_schedule(int32_t pop, int32_t conv, const ic_dirvec *dv, ic_mov
   }
 }
 
-static int lanes_of(int32_t n) {
-  int l = 1;
-  while (l < n && l < 64) l <<= 1;
-  return l;
+/* The lanes of one evaluation and the schedule periods each lane holds.
+ * An iteration's evaluations fill only a few wavefronts per SIMD, so a
+ * launch lasts about as long as its slowest wavefront: 2 periods a lane (1
+ * for a one-period swaption) measured fastest on the reference's datasets
+ * (against 1, 4 and 8), up to 128 periods; longer schedules take 4 or 8
+ * periods a lane in the second launch. */
+static void layout_of(int32_t n, int32_t *lanes, int32_t *per_lane) {
+  const int32_t per = n <= 1 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : 8;
+  int32_t l = 1;
+  while (l * per < n) l <<= 1;
+  *lanes = l;
+  *per_lane = per;
 }
 
 /* Everything one call keeps on the device. */
@@ -72,8 +81,9 @@ typedef struct {
   int32_t nswap, pop, nherm;
   ic_swaption *sw;
   ic_period *per;
-  ic_work *work; /* pop's K=1 list, pop's K=8 list, then the same for one genome */
-  int32_t nw1, nw8, nw1_one, nw8_one;
+  ic_work *work;          /* the work lists below, back to back */
+  ic_worklists pop_lists; /* for `pop` genomes */
+  ic_worklists one_lists; /* for one genome (the winner's summary) */
   float *state;  /* genomes, props [pop][5]; logliks, fb [pop]; out [8], winner [8] */
   float *terms;  /* [nswap][pop] */
   size_t sw_bytes, per_bytes, work_bytes, state_bytes, terms_bytes;
@@ -86,6 +96,20 @@ static void call_release(struct futhark_context *ctx, ic_call *c) {
   if (c->state) lvc_dev_free(ctx, c->state, c->state_bytes);
   if (c->terms) lvc_dev_free(ctx, c->terms, c->terms_bytes);
   memset(c, 0, sizeof(*c));
+}
+
+/* Insertion sort of swaption indices by descending schedule length, ties
+ * in index order (stable). */
+static void sort_by_periods(int64_t *order, int64_t n, const ic_swaption *sw) {
+  for (int64_t i = 1; i < n; i++) {
+    const int64_t v = order[i];
+    int64_t j = i;
+    while (j > 0 && sw[order[j - 1]].n < sw[v].n) {
+      order[j] = order[j - 1];
+      j--;
+    }
+    order[j] = v;
+  }
 }
 
 static int64_t add_work(ic_work *w, int64_t k, int32_t s, int32_t lanes, int32_t npop) {
@@ -130,7 +154,7 @@ static int call_prepare(struct futhark_context *ctx, ic_call *c, const lvc_ndarr
                                              hipMemcpyDeviceToHost, ctx->stream),
                          "hipMemcpyAsync") ||
            lvc_check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  int64_t nper = 0, nw1 = 0, nw8 = 0, nw1_one = 0, nw8_one = 0;
+  int64_t nper = 0, nwp[IC_LAUNCHES] = {0}, nwo[IC_LAUNCHES] = {0};
   for (int64_t s = 0; !rc && s < nswap; s++) {
     const float *row = host + s * cols;
     const float periods = 12.0f * row[2] / row[1]; /* i64.f32 (:491) */
@@ -144,38 +168,54 @@ static int call_prepare(struct futhark_context *ctx, ic_call *c, const lvc_ndarr
       rc = 1;
     } else {
       const int32_t n = (int32_t)(int64_t)periods;
+      int32_t lanes, per;
+      layout_of(n, &lanes, &per);
       sw[s].n = n;
       sw[s].off = (int32_t)nper;
-      sw[s].lanes = lanes_of(n);
+      sw[s].lanes = lanes;
+      sw[s].per_lane = per;
       nper += n;
-      const int64_t waves = (pop + 64 / sw[s].lanes - 1) / (64 / sw[s].lanes);
-      if (n <= 64) {
-        nw1 += waves;
-        nw1_one += 1;
-      } else {
-        nw8 += waves;
-        nw8_one += 1;
-      }
+      nwp[per > 2] += (pop + 64 / lanes - 1) / (64 / lanes);
+      nwo[per > 2] += 1;
     }
   }
   ic_work *work = NULL;
+  int64_t *order = NULL;
+  int64_t total = 0;
+  for (int cl = 0; cl < IC_LAUNCHES; cl++) total += nwp[cl] + nwo[cl];
   if (!rc) {
-    work = malloc((size_t)(nw1 + nw8 + nw1_one + nw8_one) * sizeof(ic_work));
+    work = malloc((size_t)(total > 0 ? total : 1) * sizeof(ic_work));
     if (!work) {
       lvc_set_error(ctx, "host allocation failed");
       rc = 1;
     }
   }
   if (!rc) {
+    order = malloc((size_t)nswap * sizeof(int64_t));
+    if (!order) {
+      lvc_set_error(ctx, "host allocation failed");
+      rc = 1;
+    }
+  }
+  int64_t start[2][IC_LAUNCHES];
+  if (!rc) {
+    /* the longest schedules first: their wavefronts run longest */
+    for (int64_t s = 0; s < nswap; s++) order[s] = s;
+    sort_by_periods(order, nswap, sw);
     int64_t k = 0;
-    for (int pass = 0; pass < 4; pass++) /* pop K=1, pop K=8, one K=1, one K=8 */
-      for (int64_t s = 0; s < nswap; s++)
-        if ((sw[s].n <= 64) == (pass % 2 == 0))
-          k = add_work(work, k, (int32_t)s, sw[s].lanes, pass < 2 ? pop : 1);
-    c->nw1 = (int32_t)nw1;
-    c->nw8 = (int32_t)nw8;
-    c->nw1_one = (int32_t)nw1_one;
-    c->nw8_one = (int32_t)nw8_one;
+    for (int one = 0; one < 2; one++) /* pop's lists by launch, then one genome's */
+      for (int cl = 0; cl < IC_LAUNCHES; cl++) {
+        start[one][cl] = k;
+        for (int64_t r = 0; r < nswap; r++) {
+          const int64_t s = order[r];
+          if ((sw[s].per_lane > 2) == cl)
+            k = add_work(work, k, (int32_t)s, sw[s].lanes, one ? 1 : pop);
+        }
+      }
+    for (int cl = 0; cl < IC_LAUNCHES; cl++) {
+      c->pop_lists.nwork[cl] = (int32_t)nwp[cl];
+      c->one_lists.nwork[cl] = (int32_t)nwo[cl];
+    }
     c->sw_bytes = (size_t)nswap * sizeof(ic_swaption);
     c->per_bytes = (size_t)(nper > 0 ? nper : 1) * sizeof(ic_period);
     c->work_bytes = (size_t)k * sizeof(ic_work);
@@ -183,6 +223,10 @@ static int call_prepare(struct futhark_context *ctx, ic_call *c, const lvc_ndarr
          lvc_dev_alloc(ctx, c->per_bytes, (void **)&c->per) ||
          lvc_dev_alloc(ctx, c->work_bytes, (void **)&c->work) ||
          lvc_upload(ctx, c->sw, sw, c->sw_bytes) || lvc_upload(ctx, c->work, work, c->work_bytes);
+    for (int cl = 0; !rc && cl < IC_LAUNCHES; cl++) {
+      c->pop_lists.work[cl] = c->work + start[0][cl];
+      c->one_lists.work[cl] = c->work + start[1][cl];
+    }
   }
   if (!rc)
     rc = lvc_check_hip(ctx, ic_launch_setup((void *)ctx->stream, swp->mem, cols, (int32_t)nswap,
@@ -191,6 +235,7 @@ static int call_prepare(struct futhark_context *ctx, ic_call *c, const lvc_ndarr
   free(host);
   free(sw);
   free(work);
+  free(order);
   return rc;
 }
 
@@ -250,18 +295,17 @@ static int run_calibration(struct futhark_context *ctx, ic_call *c, int32_t conv
   float *genomes = c->state, *props = genomes + 5 * (size_t)pop;
   float *logliks = props + 5 * (size_t)pop, *fb = logliks + pop;
   float *out = fb + pop, *winner = out + 8;
-  const ic_work *w1 = c->work, *w8 = w1 + c->nw1, *w1o = w8 + c->nw8, *w8o = w1o + c->nw1_one;
   const ic_move none = {0, 0, 0, 0};
   int tok = lvc_prof_begin(ctx, "ic_calibration");
   int e = ic_launch_init(st, pop, dv, genomes, props);
   if (!e)
-    e = ic_launch_eval(st, c->sw, c->per, w1, c->nw1, w8, c->nw8, genomes, pop, hx, hw, c->nherm,
+    e = ic_launch_eval(st, c->sw, c->per, &c->pop_lists, genomes, pop, hx, hw, c->nherm,
                        c->nswap, IC_OUT_TERMS, c->terms, NULL);
   if (!e)
     e = ic_launch_step(st, pop, c->nswap, c->terms, genomes, props, logliks, fb, 1, 0,
                        conv > 0 ? &moves[0] : &none, dv, -1, conv, trace);
   for (int32_t it = 0; !e && it < conv; it++) {
-    e = ic_launch_eval(st, c->sw, c->per, w1, c->nw1, w8, c->nw8, props, pop, hx, hw, c->nherm,
+    e = ic_launch_eval(st, c->sw, c->per, &c->pop_lists, props, pop, hx, hw, c->nherm,
                        c->nswap, IC_OUT_TERMS, c->terms, NULL);
     if (!e)
       e = ic_launch_step(st, pop, c->nswap, c->terms, genomes, props, logliks, fb, 0, acc[it],
@@ -269,8 +313,8 @@ static int run_calibration(struct futhark_context *ctx, ic_call *c, int32_t conv
   }
   if (!e) e = ic_launch_winner(st, pop, genomes, logliks, out, winner);
   if (!e && c->nswap > 0)
-    e = ic_launch_eval(st, c->sw, c->per, w1o, c->nw1_one, w8o, c->nw8_one, winner, 1, hx, hw,
-                       c->nherm, c->nswap, IC_OUT_SUMMARY, summary, NULL);
+    e = ic_launch_eval(st, c->sw, c->per, &c->one_lists, winner, 1, hx, hw, c->nherm, c->nswap,
+                       IC_OUT_SUMMARY, summary, NULL);
   lvc_prof_end(ctx, tok);
   return lvc_check_hip(ctx, e, "InterestCalib kernels");
 }
@@ -504,14 +548,13 @@ int interestcalib_eval(struct futhark_context *ctx, const int32_t ngenomes, cons
     const ic_move none = {0, 0, 0, 0};
     ic_dirvec dv;
     memset(&dv, 0, sizeof(dv));
-    const ic_work *w8 = c.work + c.nw1;
     int e = 0;
     if (ns > 0) {
-      e = ic_launch_eval(st, c.sw, c.per, c.work, c.nw1, w8, c.nw8, c.state, ngenomes, hx, hw,
-                         c.nherm, c.nswap, IC_OUT_PRICES, pq, pq + np * ns);
+      e = ic_launch_eval(st, c.sw, c.per, &c.pop_lists, c.state, ngenomes, hx, hw, c.nherm,
+                         c.nswap, IC_OUT_PRICES, pq, pq + np * ns);
       if (!e)
-        e = ic_launch_eval(st, c.sw, c.per, c.work, c.nw1, w8, c.nw8, c.state, ngenomes, hx, hw,
-                           c.nherm, c.nswap, IC_OUT_TERMS, c.terms, NULL);
+        e = ic_launch_eval(st, c.sw, c.per, &c.pop_lists, c.state, ngenomes, hx, hw, c.nherm,
+                           c.nswap, IC_OUT_TERMS, c.terms, NULL);
     }
     if (!e)
       e = ic_launch_step(st, ngenomes, c.nswap, c.terms, c.state, c.state + 5 * np, lg,

@@ -6,20 +6,20 @@
 #include <stdint.h>
 
 #define IC_MAX_PERIODS 512 /* schedule periods of one swaption (12 * term / freq) */
+#define IC_LAUNCHES 2      /* evaluation launches: 1-2 periods a lane, 4-8 periods a lane */
 #define IC_MAX_BITS 31     /* Sobol direction-vector bits (sobolInd's i32 shifts) */
-#define IC_STEP_THREADS 1024
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 
-/* One swaption quote.  The host fills n, off and lanes; ic_launch_setup
+/* One swaption quote.  The host fills n, off, lanes and per_lane; ic_launch_setup
  * fills the rest, which does not depend on the genome. */
 typedef struct {
-  int32_t n;     /* schedule periods */
-  int32_t off;   /* its first period in the period table */
-  int32_t lanes; /* lanes per evaluation: the power of two >= n up to 64; 64 above */
-  int32_t pad0;
+  int32_t n;        /* schedule periods */
+  int32_t off;      /* its first period in the period table */
+  int32_t lanes;    /* lanes per evaluation (a power of two, 1..64) */
+  int32_t per_lane; /* periods per lane (1, 2, 4 or 8); lanes * per_lane >= n */
   float tmat0;   /* date_act_365(maturity, today) */
   float zc_mat;  /* zc(maturity) */
   float quote;   /* the Black price (:509-525) */
@@ -37,6 +37,14 @@ typedef struct {
 typedef struct {
   int32_t swaption, genome0;
 } ic_work;
+
+/* The wavefronts of the evaluation launches: [0] swaptions of 1 or 2
+ * periods a lane, [1] of 4 or 8 (each wavefront's class is its swaption's
+ * per_lane). */
+typedef struct {
+  const ic_work *work[IC_LAUNCHES];
+  int32_t nwork[IC_LAUNCHES];
+} ic_worklists;
 
 /* The Sobol direction vector (:190-202). */
 typedef struct {
@@ -64,7 +72,7 @@ typedef struct {
 enum { IC_OUT_TERMS = 0, IC_OUT_PRICES = 1, IC_OUT_SUMMARY = 2 };
 
 /* Genome-independent swaption data: nswap rows of `stride` floats at
- * swaptions; sw[s].n/off/lanes set.  Returns 0 or a hipError_t value. */
+ * swaptions; sw[s].n, off, lanes and per_lane set.  Returns 0 or a hipError_t value. */
 int ic_launch_setup(void *stream, const float *swaptions, int64_t stride, int32_t nswap,
                     ic_swaption *sw, ic_period *periods);
 
@@ -72,20 +80,22 @@ int ic_launch_setup(void *stream, const float *swaptions, int64_t stride, int32_
 int ic_launch_init(void *stream, int32_t pop, const ic_dirvec *dv, float *genomes, float *props);
 
 /* evalGenomeOnSwap (:484-602) of genomes [npop][5] on the swaptions of the
- * work lists (work1: swaptions of at most 64 periods; work8: 65..512).
+ * work lists.
  * IC_OUT_TERMS: out0 [nswap][npop] the log-likelihood terms (:606-625);
  * IC_OUT_PRICES: out0, out1 [npop][nswap] quotes and prices;
  * IC_OUT_SUMMARY (npop 1): out0 [nswap][3] makeSummary's rows (:205-213). */
 int ic_launch_eval(void *stream, const ic_swaption *sw, const ic_period *periods,
-                   const ic_work *work1, int32_t nwork1, const ic_work *work8, int32_t nwork8,
-                   const float *genomes, int32_t npop, const float *herm_x, const float *herm_w,
-                   int32_t nherm, int32_t nswap, int mode, float *out0, float *out1);
+                   const ic_worklists *wl, const float *genomes, int32_t npop,
+                   const float *herm_x, const float *herm_w, int32_t nherm, int32_t nswap,
+                   int mode, float *out0, float *out1);
 
-/* One workgroup: the log-likelihood of each genome, the sum of its terms
- * over the swaptions in order; then, with init, logliks := those sums,
- * else the accept step (:733-745) with draws from Sobol offset acc; then
- * the proposals of move `next` into props (ratios into fb).  it is the
- * iteration whose accept step this is (-1 for init), for the trace. */
+/* Two launches: the log-likelihood of each genome, the sum of its terms
+ * over the swaptions in a fixed order (32 threads per genome each sum the
+ * swaptions t, t + 32, ... in order, and the partial sums are added
+ * pairwise), then, with init, logliks := those sums, else the accept step
+ * (:733-745) with draws from Sobol offset acc; then, unless next is no
+ * move, the proposals of move `next` into props (ratios into fb).  it is
+ * the iteration whose accept step this is (-1 for init), for the trace. */
 int ic_launch_step(void *stream, int32_t pop, int32_t nswap, const float *terms, float *genomes,
                    float *props, float *logliks, float *fb, int init, int32_t acc,
                    const ic_move *next, const ic_dirvec *dv, int32_t it, int32_t conv,

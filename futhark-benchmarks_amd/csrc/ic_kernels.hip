@@ -5,21 +5,27 @@
 // model price from three Brent root finds over the swaption's schedule and
 // a Gauss-Hermite sum over it.  That work is independent per (genome,
 // swaption) and bounded by the schedule length n (2..60 periods in the
-// datasets), so one evaluation runs on a group of lanes, one schedule period
-// per lane: the group is n rounded up to a power of two (2..64 lanes; above
-// 64 periods 64 lanes of up to 8 periods each), a wavefront holds 64/lanes
-// evaluations of the same swaption for consecutive genomes, and every sum,
-// maximum and Brent iteration over the schedule is a butterfly within the
-// group.  The butterfly hands every lane of the group the same bits, so a
-// group's control flow (the root finder's branches and iteration counts)
-// stays uniform within the group.
+// datasets), so one evaluation runs on a group of lanes holding K schedule
+// periods each (K = 1, 2, 4 or 8, a template parameter; the group is a power
+// of two of 1..64 lanes), a wavefront holds 64/lanes evaluations of the same
+// swaption for consecutive genomes, and every sum, maximum and Brent
+// iteration over the schedule is a sum over the lane's K periods followed by
+// a butterfly within the group.  The butterfly hands every lane of the group
+// the same bits, so a group's control flow (the root finder's branches and
+// iteration counts) stays uniform within the group.  An iteration's
+// evaluations (25-50K on the datasets) fill only a few wavefronts per SIMD,
+// so a launch lasts about as long as its slowest wavefront: the host keeps
+// K at 2 (1 for one-period swaptions) up to 128 periods, which measured
+// faster than packing more periods into fewer, longer-running lanes, and
+// every class of such swaptions runs in one launch.
 //
 // What does not depend on the genome (dates, discount factors, the Black
 // quote, each period's c_i * zc(end) / zc(maturity)) is computed once per
 // call by ic_setup_kernel into per-swaption and per-period tables.  The
-// log-likelihood terms land in [swaption][genome] order and one workgroup
-// sums them per genome in swaption order, accepts or rejects, and draws the
-// next proposals (ic_step_kernel), so an iteration is two launches.
+// log-likelihood terms land in [swaption][genome] order; ic_accept_kernel
+// sums them per genome in a fixed order and accepts or rejects, and
+// ic_propose_kernel draws the next proposals, so an iteration is three
+// launches, each spread over the device.
 //
 // Arithmetic is f32 in the reference's order of operations, compiled
 // without contraction; expf/logf/sqrtf are the device library's, so results
@@ -148,13 +154,16 @@ __device__ inline float log_likelihood(float y_ref, float y) {  // Cauchy, :606-
 }
 
 // ---- Sobol numbers (:190-202) ------------------------------------------------
+// Unrolled over the largest vector so the direction numbers load once, up
+// front, rather than one scalar load and wait per bit.
 __device__ inline float sobol(const ic_dirvec &dv, int32_t n) {
   const float norm_fact = 1.0f / ((float)((int64_t)1 << dv.bits) + 1.0f);
   const int32_t n_gray = (n >> 1) ^ n;
   int32_t res = 0;
-  for (int i = 0; i < dv.bits; i++) {
+#pragma unroll
+  for (int i = 0; i < IC_MAX_BITS; i++) {
     const int32_t t = (int32_t)(1u << i);
-    if ((n_gray & t) == t) res ^= dv.dir[i];
+    if (i < dv.bits && (n_gray & t) == t) res ^= dv.dir[i];
   }
   return (float)res * norm_fact;
 }
@@ -349,35 +358,56 @@ __device__ float exact_yhat(const Periods<K> &P, int n, int lanes, const Scals &
 }  // namespace
 
 // ---- per-call setup: what does not depend on the genome (:486-525, :533-547)
-__global__ void ic_setup_kernel(const float *__restrict__ swp, int64_t stride, int32_t nswap,
-                                ic_swaption *__restrict__ sw, ic_period *__restrict__ per) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per swaption, its periods spread over the threads; the
+// quote's level is summed in period order by one thread, as the reference's
+// sequential reduction.
+constexpr int kSetupThreads = 64;
+
+__global__ __launch_bounds__(kSetupThreads) void ic_setup_kernel(
+    const float *__restrict__ swp, int64_t stride, int32_t nswap, ic_swaption *__restrict__ sw,
+    ic_period *__restrict__ per) {
+  __shared__ float a12s[IC_MAX_PERIODS];
+  __shared__ int32_t a1s[IC_MAX_PERIODS], a2s[IC_MAX_PERIODS];
+  __shared__ float strike_sh;
+  const int32_t s = blockIdx.x;
   if (s >= nswap) return;
   const float *row = swp + (int64_t)s * stride;
-  ic_swaption S = sw[s];
+  const ic_swaption S0 = sw[s];
+  const int32_t n = S0.n;
   const int32_t t_today = today();
   const float swap_freq = row[1];
   const int32_t maturity = add_months(t_today, (int32_t)(row[0] * 12.0f));  // add_years, :33
   const float tmat0 = act365(maturity, t_today);
-  const int32_t n = S.n;
-  float lvl = 0.0f;
-  int32_t t0 = date_of_gregorian(2229, 12, 31, 12, 0), tn = date_of_gregorian(1980, 1, 1, 12, 0);
-  for (int32_t i = 0; i < n; i++) {
+  for (int32_t i = threadIdx.x; i < n; i += kSetupThreads) {
     const int32_t a1 = add_months(maturity, (int32_t)(swap_freq * (float)(int64_t)i));
     const int32_t a2 = add_months(a1, (int32_t)swap_freq);
-    lvl = lvl + zc(a2) * act365(a2, a1);
-    t0 = t0 < a1 ? t0 : a1;
-    tn = tn > a2 ? tn : a2;
+    a1s[i] = a1;
+    a2s[i] = a2;
+    a12s[i] = zc(a2) * act365(a2, a1);
   }
-  const float strike = (zc(t0) - zc(tn)) / lvl;
-  const float d1 = 0.5f * row[3] * tmat0;
-  S.quote = lvl * strike * (ugaussian_P(d1) - ugaussian_P(-d1));
-  S.tmat0 = tmat0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float lvl = 0.0f;
+    int32_t t0 = date_of_gregorian(2229, 12, 31, 12, 0), tn = date_of_gregorian(1980, 1, 1, 12, 0);
+    for (int32_t i = 0; i < n; i++) {
+      lvl = lvl + a12s[i];
+      t0 = t0 < a1s[i] ? t0 : a1s[i];
+      tn = tn > a2s[i] ? tn : a2s[i];
+    }
+    const float strike = (zc(t0) - zc(tn)) / lvl;
+    const float d1 = 0.5f * row[3] * tmat0;
+    ic_swaption S = S0;
+    S.quote = lvl * strike * (ugaussian_P(d1) - ugaussian_P(-d1));
+    S.tmat0 = tmat0;
+    S.zc_mat = zc(maturity);
+    sw[s] = S;
+    strike_sh = strike;
+  }
+  __syncthreads();
+  const float strike = strike_sh;
   const float zc_mat = zc(maturity);
-  S.zc_mat = zc_mat;
-  for (int32_t i = 0; i < n; i++) {
-    const int32_t beg = add_months(maturity, (int32_t)(swap_freq * (float)(int64_t)i));
-    const int32_t end = add_months(beg, (int32_t)swap_freq);
+  for (int32_t i = threadIdx.x; i < n; i += kSetupThreads) {
+    const int32_t beg = a1s[i], end = a2s[i];  // the same dates as the quote's (:533-536)
     const float res = act365(end, beg) * strike;
     const float cii = i == n - 1 ? 1.0f + res : res;
     ic_period p;
@@ -385,9 +415,8 @@ __global__ void ic_setup_kernel(const float *__restrict__ swp, int64_t stride, i
     p.tod2 = act365(end, maturity);
     p.fact = cii * zc(end) / zc_mat;
     p.log_fact = logf(p.fact);
-    per[S.off + i] = p;
+    per[S0.off + i] = p;
   }
-  sw[s] = S;
 }
 
 __global__ void ic_init_kernel(int32_t pop, ic_dirvec dv, float *__restrict__ genomes,
@@ -415,13 +444,10 @@ struct EvalArgs {
   float *out0, *out1;
 };
 
+// One wavefront's evaluations, K schedule periods a lane.
 template <int K>
-__global__ __launch_bounds__(256) void ic_eval_kernel(EvalArgs A) {
-  const int lane = threadIdx.x & 63;
-  const int32_t wv = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (wv >= A.nwork) return;  // wave-uniform
-  const ic_work wk = A.work[wv];
-  const ic_swaption S = A.sw[wk.swaption];
+__device__ __forceinline__ void eval_wave(const EvalArgs &A, const ic_work wk, const ic_swaption S,
+                                          const int lane) {
   const int lanes = S.lanes;
   const int n = S.n;
   const int j = lane & (lanes - 1);
@@ -526,6 +552,31 @@ __global__ __launch_bounds__(256) void ic_eval_kernel(EvalArgs A) {
   }
 }
 
+// Swaptions of 1 or 2 periods a lane (up to 128 periods) in one launch, so
+// the classes share the device; a class is uniform per wavefront.
+__global__ __launch_bounds__(256) void ic_eval_kernel(EvalArgs A) {
+  const int32_t wv = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (wv >= A.nwork) return;  // wave-uniform
+  const ic_work wk = A.work[wv];
+  const ic_swaption S = A.sw[wk.swaption];
+  if (S.per_lane == 1)
+    eval_wave<1>(A, wk, S, threadIdx.x & 63);
+  else
+    eval_wave<2>(A, wk, S, threadIdx.x & 63);
+}
+
+// Longer schedules (4 or 8 periods a lane, up to 512 periods): more registers.
+__global__ __launch_bounds__(256) void ic_eval_long_kernel(EvalArgs A) {
+  const int32_t wv = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (wv >= A.nwork) return;
+  const ic_work wk = A.work[wv];
+  const ic_swaption S = A.sw[wk.swaption];
+  if (S.per_lane == 4)
+    eval_wave<4>(A, wk, S, threadIdx.x & 63);
+  else
+    eval_wave<8>(A, wk, S, threadIdx.x & 63);
+}
+
 // ---- accept / propose ------------------------------------------------------------------
 struct StepArgs {
   int32_t pop, nswap;
@@ -537,86 +588,110 @@ struct StepArgs {
   ic_trace trace;
 };
 
-__global__ __launch_bounds__(IC_STEP_THREADS) void ic_step_kernel(StepArgs A) {
+constexpr int kSumLanes = 32;  // threads per genome of the accept kernel
+
+// The log-likelihood of each genome (:639-645) and the accept step
+// (:733-745).  Accepting needs only the genome's own terms, so the genomes
+// spread over the device: 32 lanes per genome, lane t summing the swaptions
+// t, t + 32, ... in order (one memory round trip on the datasets), the 32
+// partial sums added pairwise with the same bits on every lane.
+__global__ __launch_bounds__(256) void ic_accept_kernel(StepArgs A) {
   const int32_t pop = A.pop;
-  for (int32_t i = threadIdx.x; i < pop; i += blockDim.x) {
-    float s = 0.0f;  // the genome's log-likelihood (:639-645), in swaption order
-    for (int32_t w = 0; w < A.nswap; w++) s += A.terms[(int64_t)w * pop + i];
-    if (A.init) {
-      A.logliks[i] = s;
-    } else {  // :733-745
-      const float acceptance = fminf(1.0f, expf(s - A.logliks[i]) * A.fb[i]);
-      const float r = sobol(A.dv, A.acc + i);
-      if (A.trace.newl) A.trace.newl[(int64_t)A.it * pop + i] = s;
-      if (r < acceptance) {
-        for (int j = 0; j < 5; j++) A.genomes[5 * i + j] = A.props[5 * i + j];
-        A.logliks[i] = s;
-      }
+  const int t = threadIdx.x & (kSumLanes - 1);
+  const int32_t i = (int32_t)(blockIdx.x * (256 / kSumLanes) + threadIdx.x / kSumLanes);
+  float s = 0.0f;
+  if (i < pop) {
+    const float *col = A.terms + i;
+    int32_t w = t;
+    for (; w + 7 * kSumLanes < A.nswap; w += 8 * kSumLanes) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = col[(int64_t)(w + k * kSumLanes) * pop];
+#pragma unroll
+      for (int k = 0; k < 8; k++) s += v[k];
     }
-    if (A.trace.genomes) {
-      const int64_t row = (int64_t)(A.it + 1) * pop + i;
-      for (int j = 0; j < 5; j++) A.trace.genomes[5 * row + j] = A.genomes[5 * i + j];
-      A.trace.logliks[row] = A.logliks[i];
+    for (; w < A.nswap; w += kSumLanes) s += col[(int64_t)w * pop];
+  }
+  for (int m = 1; m < kSumLanes; m <<= 1) s += __shfl_xor(s, m, 64);
+  if (i >= pop || t != 0) return;
+  if (A.init) {
+    A.logliks[i] = s;
+  } else {
+    const float acceptance = fminf(1.0f, expf(s - A.logliks[i]) * A.fb[i]);
+    const float r = sobol(A.dv, A.acc + i);
+    if (A.trace.newl) A.trace.newl[(int64_t)A.it * pop + i] = s;
+    if (r < acceptance) {
+      for (int j = 0; j < 5; j++) A.genomes[5 * i + j] = A.props[5 * i + j];
+      A.logliks[i] = s;
     }
   }
-  if (A.next.type == 0) return;
-  __syncthreads();  // DE reads other genomes' accepted values
+  if (A.trace.genomes) {
+    const int64_t row = (int64_t)(A.it + 1) * pop + i;
+    for (int j = 0; j < 5; j++) A.trace.genomes[5 * row + j] = A.genomes[5 * i + j];
+    A.trace.logliks[row] = A.logliks[i];
+  }
+}
+
+// The next iteration's proposals (:655-724), one thread per genome, after
+// every genome's accept step (differential evolution reads other genomes).
+__global__ __launch_bounds__(256) void ic_propose_kernel(StepArgs A) {
+  const int32_t pop = A.pop;
+  const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+  if (i >= pop) return;
   const ic_move mv = A.next;
-  for (int32_t i = threadIdx.x; i < pop; i += blockDim.x) {
-    float out[5], fb = 1.0f;
-    if (mv.type == 1 || mv.type == 2) {  // mutate_dims_all / mutate_dims_one (:278-306)
-      for (int j = 0; j < 5; j++) {
-        const float r01 = sobol(A.dv, 5 * i + mv.base + j);
-        const float ampl = mv.type == 1 || j == mv.dim ? kAmplRatio : 0.0f;
-        const float g_min = gene_min(j), g_max = gene_max(j);
-        const float gene = A.genomes[5 * i + j], prop = A.props[5 * i + j];
-        const float amplitude = fabsf((g_max - g_min) * ampl);  // mutateHelper, :250-266
-        const float semiamplitude = amplitude / 2.0f;
-        float tmp_min_max = fminf(g_max, gene + semiamplitude);
-        float tmp_max_min = fmaxf(g_min, gene - semiamplitude);
-        const float forward_range = tmp_min_max - tmp_max_min;
-        tmp_min_max = fminf(g_max, prop + semiamplitude);
-        tmp_max_min = fmaxf(g_min, prop - semiamplitude);
-        const float backward_range = tmp_min_max - tmp_max_min;
-        const float bf = 0.0f < semiamplitude ? backward_range / forward_range : 1.0f;
-        const float ng = gene + amplitude * r01 - semiamplitude;
-        out[j] = fmaxf(g_min, fminf(g_max, ng));  // constrainDim, :268-270
-        fb = j == 0 ? bf : fb * bf;
-      }
-    } else {  // DE (:700-723, mcmc_DE :309-320)
-      const int32_t kk = 8 * i + mv.base;
-      const float s1 = sobol(A.dv, kk);
-      int32_t k = (int32_t)(s1 * (float)(pop - 1));
-      int32_t cand_ub = pop - 1;
-      if (k == i) {
-        k = pop - 1;
-        cand_ub = pop - 2;
-      }
-      const float s2 = sobol(A.dv, kk + 1);
-      int32_t l = (int32_t)(s2 * (float)cand_ub);
-      if (l == i || l == k) l = cand_ub;
-      const float s3 = sobol(A.dv, kk + 2);
-      const float gamma_avg = 2.38f / sqrtf(2.0f * 5.0f);
-      const float ampl_ratio = 0.1f * kAmplRatio;
-      const float gamma1 = gamma_avg - 0.5f + s3;
-      for (int j = 0; j < 5; j++) {
-        const float r01 = sobol(A.dv, kk + 3 + j);
-        const float mm_diff = gene_max(j) - gene_min(j);
-        const float amplitude = fabsf(mm_diff * ampl_ratio);
-        const float semiamplitude = amplitude / 2.0f;
-        const float perturb = amplitude * r01 - semiamplitude;
-        const float ng = A.genomes[5 * i + j] + perturb +
-                         gamma1 * (A.genomes[5 * k + j] - A.genomes[5 * l + j]);
-        out[j] = fmaxf(gene_min(j), fminf(gene_max(j), ng));
-      }
+  float out[5], fb = 1.0f;
+  if (mv.type == 1 || mv.type == 2) {  // mutate_dims_all / mutate_dims_one (:278-306)
+    for (int j = 0; j < 5; j++) {
+      const float r01 = sobol(A.dv, 5 * i + mv.base + j);
+      const float ampl = mv.type == 1 || j == mv.dim ? kAmplRatio : 0.0f;
+      const float g_min = gene_min(j), g_max = gene_max(j);
+      const float gene = A.genomes[5 * i + j], prop = A.props[5 * i + j];
+      const float amplitude = fabsf((g_max - g_min) * ampl);  // mutateHelper, :250-266
+      const float semiamplitude = amplitude / 2.0f;
+      float tmp_min_max = fminf(g_max, gene + semiamplitude);
+      float tmp_max_min = fmaxf(g_min, gene - semiamplitude);
+      const float forward_range = tmp_min_max - tmp_max_min;
+      tmp_min_max = fminf(g_max, prop + semiamplitude);
+      tmp_max_min = fmaxf(g_min, prop - semiamplitude);
+      const float backward_range = tmp_min_max - tmp_max_min;
+      const float bf = 0.0f < semiamplitude ? backward_range / forward_range : 1.0f;
+      const float ng = gene + amplitude * r01 - semiamplitude;
+      out[j] = fmaxf(g_min, fminf(g_max, ng));  // constrainDim, :268-270
+      fb = j == 0 ? bf : fb * bf;
     }
-    for (int j = 0; j < 5; j++) A.props[5 * i + j] = out[j];
-    A.fb[i] = fb;
-    if (A.trace.props && A.it + 1 < A.conv) {
-      const int64_t row = (int64_t)(A.it + 1) * pop + i;
-      for (int j = 0; j < 5; j++) A.trace.props[5 * row + j] = out[j];
-      A.trace.fb[row] = fb;
+  } else {  // DE (:700-723, mcmc_DE :309-320)
+    const int32_t kk = 8 * i + mv.base;
+    const float s1 = sobol(A.dv, kk);
+    int32_t k = (int32_t)(s1 * (float)(pop - 1));
+    int32_t cand_ub = pop - 1;
+    if (k == i) {
+      k = pop - 1;
+      cand_ub = pop - 2;
     }
+    const float s2 = sobol(A.dv, kk + 1);
+    int32_t l = (int32_t)(s2 * (float)cand_ub);
+    if (l == i || l == k) l = cand_ub;
+    const float s3 = sobol(A.dv, kk + 2);
+    const float gamma_avg = 2.38f / sqrtf(2.0f * 5.0f);
+    const float ampl_ratio = 0.1f * kAmplRatio;
+    const float gamma1 = gamma_avg - 0.5f + s3;
+    for (int j = 0; j < 5; j++) {
+      const float r01 = sobol(A.dv, kk + 3 + j);
+      const float mm_diff = gene_max(j) - gene_min(j);
+      const float amplitude = fabsf(mm_diff * ampl_ratio);
+      const float semiamplitude = amplitude / 2.0f;
+      const float perturb = amplitude * r01 - semiamplitude;
+      const float ng = A.genomes[5 * i + j] + perturb +
+                       gamma1 * (A.genomes[5 * k + j] - A.genomes[5 * l + j]);
+      out[j] = fmaxf(gene_min(j), fminf(gene_max(j), ng));
+    }
+  }
+  for (int j = 0; j < 5; j++) A.props[5 * i + j] = out[j];
+  A.fb[i] = fb;
+  if (A.trace.props && A.it + 1 < A.conv) {
+    const int64_t row = (int64_t)(A.it + 1) * pop + i;
+    for (int j = 0; j < 5; j++) A.trace.props[5 * row + j] = out[j];
+    A.trace.fb[row] = fb;
   }
 }
 
@@ -645,8 +720,8 @@ __global__ void ic_winner_kernel(int32_t pop, const float *__restrict__ genomes,
 extern "C" int ic_launch_setup(void *stream, const float *swaptions, int64_t stride,
                                int32_t nswap, ic_swaption *sw, ic_period *periods) {
   if (nswap <= 0) return 0;
-  hipLaunchKernelGGL(ic_setup_kernel, dim3((nswap + 63) / 64), dim3(64), 0,
-                     (hipStream_t)stream, swaptions, stride, nswap, sw, periods);
+  hipLaunchKernelGGL(ic_setup_kernel, dim3(nswap), dim3(kSetupThreads), 0, (hipStream_t)stream,
+                     swaptions, stride, nswap, sw, periods);
   return (int)hipGetLastError();
 }
 
@@ -659,8 +734,7 @@ extern "C" int ic_launch_init(void *stream, int32_t pop, const ic_dirvec *dv, fl
 }
 
 extern "C" int ic_launch_eval(void *stream, const ic_swaption *sw, const ic_period *periods,
-                              const ic_work *work1, int32_t nwork1, const ic_work *work8,
-                              int32_t nwork8, const float *genomes, int32_t npop,
+                              const ic_worklists *wl, const float *genomes, int32_t npop,
                               const float *herm_x, const float *herm_w, int32_t nherm,
                               int32_t nswap, int mode, float *out0, float *out1) {
   EvalArgs A;
@@ -675,17 +749,15 @@ extern "C" int ic_launch_eval(void *stream, const ic_swaption *sw, const ic_peri
   A.mode = mode;
   A.out0 = out0;
   A.out1 = out1;
-  if (nwork1 > 0) {
-    A.work = work1;
-    A.nwork = nwork1;
-    hipLaunchKernelGGL(ic_eval_kernel<1>, dim3((nwork1 + 3) / 4), dim3(256), 0,
-                       (hipStream_t)stream, A);
-  }
-  if (nwork8 > 0) {
-    A.work = work8;
-    A.nwork = nwork8;
-    hipLaunchKernelGGL(ic_eval_kernel<8>, dim3((nwork8 + 3) / 4), dim3(256), 0,
-                       (hipStream_t)stream, A);
+  for (int c = IC_LAUNCHES - 1; c >= 0; c--) {
+    if (wl->nwork[c] <= 0) continue;
+    A.work = wl->work[c];
+    A.nwork = wl->nwork[c];
+    const dim3 grid((wl->nwork[c] + 3) / 4), block(256);
+    if (c == 0)
+      hipLaunchKernelGGL(ic_eval_kernel, grid, block, 0, (hipStream_t)stream, A);
+    else
+      hipLaunchKernelGGL(ic_eval_long_kernel, grid, block, 0, (hipStream_t)stream, A);
   }
   return (int)hipGetLastError();
 }
@@ -713,7 +785,13 @@ extern "C" int ic_launch_step(void *stream, int32_t pop, int32_t nswap, const fl
   } else {
     A.trace.genomes = A.trace.logliks = A.trace.props = A.trace.fb = A.trace.newl = nullptr;
   }
-  hipLaunchKernelGGL(ic_step_kernel, dim3(1), dim3(IC_STEP_THREADS), 0, (hipStream_t)stream, A);
+  if (pop <= 0) return 0;
+  const int64_t per_block = 256 / kSumLanes;
+  hipLaunchKernelGGL(ic_accept_kernel, dim3((unsigned)((pop + per_block - 1) / per_block)),
+                     dim3(256), 0, (hipStream_t)stream, A);
+  if (A.next.type != 0)
+    hipLaunchKernelGGL(ic_propose_kernel, dim3((unsigned)((pop + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, A);
   return (int)hipGetLastError();
 }
 
