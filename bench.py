@@ -338,6 +338,61 @@ def bench_option_pricing(reps=10, cpu=True):
     return res
 
 
+def bench_interest_calib(reps=5, cpu=True):
+    """finpar/InterestCalib.fut's `main` on the reference's three datasets
+    (tests/golden/InterestCalib-data), inputs resident on the device: ms per
+    call (the entry returns its six scalars to the host, so a call ends
+    synchronised), MCMC iterations and genome-swaption evaluations per
+    second; the winner's log-likelihood beside the dataset's expected one
+    (the trajectories differ in their accept decisions, see the tests).
+    With `cpu`, the oracle's rate on a bounded sample: one population's
+    log-likelihoods (pop genomes on every swaption, OpenMP, 16 threads)."""
+    import numpy as np
+    from locvolcalib import futhark_values as fv
+    from locvolcalib import interestcalib as icl
+    data = os.path.join(ROOT, "tests", "golden", "InterestCalib-data")
+    res = {}
+    with icl.InterestCalib(device=0) as ctx:
+        for name in ("small", "medium", "large"):
+            a = icl.main_arguments(fv.load(os.path.join(data, f"{name}.in")))
+            expected = float(fv.load(os.path.join(data, f"{name}.out"))[5])
+            hs = ctx.upload_inputs(a["swaptions"], a["herm_x"], a["herm_w"], a["dir_vct"])
+            try:
+                out, h = ctx.main_raw(a["pop"], a["mcmc_conv"], hs)
+                ctx.free_summary(h)
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    out, h = ctx.main_raw(a["pop"], a["mcmc_conv"], hs)
+                    ctx.free_summary(h)
+                dt = (time.perf_counter() - t0) / reps
+            finally:
+                ctx.free_inputs(hs)
+            nsw = len(a["swaptions"])
+            evals = a["pop"] * nsw * (a["mcmc_conv"] + 1)
+            r = {"pop": a["pop"], "mcmc_conv": a["mcmc_conv"], "swaptions": nsw,
+                 "ms_per_call": round(dt * 1e3, 3),
+                 "iterations_per_s": round(a["mcmc_conv"] / dt, 1),
+                 "evaluations_per_s": round(evals / dt, 1),
+                 "winner_loglik": float(out[5]), "expected_loglik": expected}
+            if cpu:
+                from oracle import oracle
+                g = np.array([[np.float32(oracle.ic_sobol(5 * i + 1 + j, a["dir_vct"]))
+                               for j in range(5)] for i in range(a["pop"])], np.float32)
+                lo = np.array([1e-3, 1e-3, np.float32(1e-3) - np.float32(1), 1e-3, 1e-3],
+                              np.float32)
+                hi = np.array([np.float32(1) - np.float32(1e-3)] * 3 + [0.2, 0.2], np.float32)
+                g = (g * (hi - lo) + lo).astype(np.float32)
+                t0 = time.perf_counter()
+                oracle.ic_logliks(g, a["swaptions"], a["herm_x"], a["herm_w"], nthreads=16)
+                ct = time.perf_counter() - t0
+                r["cpu_oracle"] = {"sample": f"the initial population's log-likelihoods "
+                                             f"({a['pop']} genomes x {nsw} swaptions)",
+                                   "evaluations_per_s": round(a["pop"] * nsw / ct, 1),
+                                   "cores": 16, "kind": "port"}
+            res[name] = r
+    return res
+
+
 def bench_serving(lvc, rate, duration_s=1.5, seed=7, max_slots=1024, budget=64 << 30):
     """Poisson arrivals at `rate` req/s (None: every request queued at time 0,
     the saturated server) of single problems (sizes and parameters as the
@@ -548,6 +603,8 @@ def main():
         out["ocean_tridiag"] = bench_ocean(lvc)
         log("OptionPricing")
         out["option_pricing"] = bench_option_pricing(cpu=check)
+        log("InterestCalib")
+        out["interest_calib"] = bench_interest_calib(cpu=check)
     if not args.no_cpu_baseline:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

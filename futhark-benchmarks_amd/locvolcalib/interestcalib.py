@@ -223,6 +223,31 @@ class InterestCalib:
                 getattr(self._L, f"futhark_free_{ty}")(self._ctx, h)
 
     # -- the entry points ---------------------------------------------------
+    def upload_inputs(self, swaptions, herm_x, herm_w, dir_vct) -> list:
+        """Device handles of main's four arrays, for :meth:`main_raw`; free
+        them with :meth:`free_inputs`."""
+        return self._inputs(swaptions, herm_x, herm_w, dir_vct)
+
+    def free_inputs(self, handles) -> None:
+        self._free_inputs(handles)
+
+    def main_raw(self, pop: int, mcmc_conv: int, handles) -> tuple:
+        """``futhark_entry_main`` on uploaded inputs: (the six scalars, the
+        summary's device handle; free it with :meth:`free_summary`)."""
+        outs = [ctypes.c_float() for _ in range(6)]
+        arr = _vp()
+        n = int(self._L.futhark_shape_f32_2d(self._ctx, handles[0][1])[0])
+        m = int(self._L.futhark_shape_f32_1d(self._ctx, handles[1][1])[0])
+        b = int(self._L.futhark_shape_i32_1d(self._ctx, handles[3][1])[0])
+        self._check(self._L.futhark_entry_main(
+            self._ctx, *[ctypes.byref(o) for o in outs], ctypes.byref(arr), int(pop),
+            int(mcmc_conv), n, handles[0][1], m, handles[1][1], handles[2][1], b,
+            handles[3][1]), "futhark_entry_main")
+        return np.array([o.value for o in outs], np.float32), arr.value
+
+    def free_summary(self, h) -> None:
+        self._L.futhark_free_f32_2d(self._ctx, h)
+
     def main(self, pop: int, mcmc_conv: int, swaptions, herm_x, herm_w, dir_vct) -> tuple:
         """InterestCalib.fut:929-936: (the six f32 scalars a, b, sigma, nu,
         rho, log-likelihood of the winner; the summary [n][3])."""

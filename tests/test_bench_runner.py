@@ -140,3 +140,43 @@ def test_runner_on_option_pricing_datasets(tmp_path):
     assert set(res) == {f"OptionPricing-data/{n}.in" for n in ("small", "medium", "large")}
     for v in res.values():
         assert len(v["runtimes"]) == 2 and min(v["runtimes"]) > 0 and v["stderr"] == ""
+
+
+# InterestCalib.fut:1-13's stanza, as data.
+IC_STANZA = """-- Interest rate calibration
+-- ==
+-- tags { nobench no_opencl }
+-- compiled input @ InterestCalib-data/small.in
+-- output @ InterestCalib-data/small.out
+--
+-- compiled input @ InterestCalib-data/medium.in
+-- output @ InterestCalib-data/medium.out
+--
+-- compiled input @ InterestCalib-data/large.in
+-- output @ InterestCalib-data/large.out
+"""
+
+
+def test_parse_interest_calib_stanza():
+    pairs = bench_runner.parse_stanza(IC_STANZA)
+    assert [p[0] for p in pairs] == [f"InterestCalib-data/{n}.in"
+                                     for n in ("small", "medium", "large")]
+    assert bench_runner.parse_entries(IC_STANZA) == ["main"]
+
+
+@pytest.mark.gpu
+def test_runner_on_interest_calib_datasets(tmp_path, capsys):
+    print("case: bench runner, InterestCalib small/medium/large", flush=True)
+    prog = tmp_path / "InterestCalib.fut"
+    prog.write_text(IC_STANZA)
+    os.symlink(os.path.join(GOLDEN, "InterestCalib-data"), tmp_path / "InterestCalib-data")
+    out = tmp_path / "o.json"
+    # large.out is a copy of medium.out, whose quotes are medium's swaptions':
+    # the runner reports that dataset's check as failed
+    assert bench_runner.main(["--program", str(prog), "--runs", "2", "--json", str(out)]) == 1
+    res = json.loads(out.read_text())[bench_runner.IC_PROGRAM_KEY]["datasets"]
+    assert set(res) == {f"InterestCalib-data/{n}.in" for n in ("small", "medium", "large")}
+    for n, v in res.items():
+        assert len(v["runtimes"]) == 2 and min(v["runtimes"]) > 0
+        assert (v["stderr"] == "") == (not n.endswith("large.in")), (n, v["stderr"])
+    assert "quote column" in res["InterestCalib-data/large.in"]["stderr"]
