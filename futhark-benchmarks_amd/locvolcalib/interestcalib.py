@@ -26,7 +26,9 @@ __all__ = ["InterestCalib", "InterestCalibError", "lib", "library_path", "export
            "main_arguments"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-library_path = os.path.join(_HERE, "libinterestcalib.so")
+# IC_LIBRARY names another build of the same sources (the tests' sanitizer
+# build, futhark-benchmarks_amd/_build/asan/libinterestcalib.so)
+library_path = os.environ.get("IC_LIBRARY") or os.path.join(_HERE, "libinterestcalib.so")
 _lib: Optional[ctypes.CDLL] = None
 
 _vp = ctypes.c_void_p

@@ -1,11 +1,12 @@
 """Compiled callers and sanitizer builds.
 
-  * The library's host C and the oracle built under AddressSanitizer and
-    UndefinedBehaviorSanitizer (`make asan` in futhark-benchmarks_amd/ and
-    oracle/), loaded into a test process that preloads gcc's libasan: the
-    oracle and ABI tests on the CPU, and a set of GPU tests (batches, the
-    server, the allocation-free entry, the tuner cache, graph replay) on the
-    GPU, the kernels as usual.
+  * The libraries' host C (LocVolCalib's and InterestCalib's) and the
+    oracle built under AddressSanitizer and UndefinedBehaviorSanitizer
+    (`make asan` in futhark-benchmarks_amd/ and oracle/), loaded into a test
+    process that preloads gcc's libasan: the oracle and ABI tests on the CPU,
+    and a set of GPU tests (batches, the server, the allocation-free entry,
+    the tuner cache, graph replay; InterestCalib's pricing, small-population
+    trajectories and error paths) on the GPU, the kernels as usual.
   * tests/native/lys_caller.c, a C program calling the library the way the
     reference's lys frontends call the generated API (shared.c:15-49,
     console/liblys.c:243-251, console/main.c:176-180), built against the
@@ -24,6 +25,7 @@ from conftest import ROOT, load_dataset
 NATIVE = os.path.join(ROOT, "tests", "native", "_build")
 ASAN_LIB = os.path.join(ROOT, "futhark-benchmarks_amd", "_build", "asan", "liblocvolcalib.so")
 ASAN_ORACLE = os.path.join(ROOT, "oracle", "_build", "liblvc_oracle_asan.so")
+ASAN_IC = os.path.join(ROOT, "futhark-benchmarks_amd", "_build", "asan", "libinterestcalib.so")
 
 
 def _libasan():
@@ -37,12 +39,12 @@ def _libasan():
 
 def _sanitized_env():
     asan = _libasan()
-    if not asan or not os.path.exists(ASAN_LIB) or not os.path.exists(ASAN_ORACLE):
+    if not asan or not all(os.path.exists(p) for p in (ASAN_LIB, ASAN_ORACLE, ASAN_IC)):
         pytest.skip("sanitizer builds or gcc's libasan missing (build with __graft_entry__.build())")
     env = dict(os.environ)
     env.update(LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
                UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1",
-               LVC_LIBRARY=ASAN_LIB, LVC_ORACLE_LIB=ASAN_ORACLE)
+               LVC_LIBRARY=ASAN_LIB, LVC_ORACLE_LIB=ASAN_ORACLE, IC_LIBRARY=ASAN_IC)
     return env
 
 
@@ -55,8 +57,8 @@ def _pytest(args, env, timeout):
 
 def test_oracle_and_abi_under_sanitizers():
     env = _sanitized_env()
-    r = _pytest(["tests/test_oracle.py", "tests/test_op_oracle.py", "tests/test_abi.py",
-                 "-m", "not gpu"], env, timeout=900)
+    r = _pytest(["tests/test_oracle.py", "tests/test_op_oracle.py", "tests/test_ic_oracle.py",
+                 "tests/test_abi.py", "-m", "not gpu"], env, timeout=900)
     assert r.returncode == 0
     assert " passed" in r.stdout and "AddressSanitizer" not in r.stderr
 
@@ -66,8 +68,9 @@ def test_gpu_paths_under_sanitizers():
     print("case: GPU tests with the host code under ASan + UBSan", flush=True)
     env = _sanitized_env()
     r = _pytest(["tests/test_gpu_parity.py", "tests/test_gpu_configs.py", "tests/test_gpu_tuning.py",
-                 "tests/test_zz_no_alloc.py", "-m", "gpu", "-k",
-                 "server or batch or budget or noalloc or autotune or graph or alone"],
+                 "tests/test_zz_no_alloc.py", "tests/test_gpu_interestcalib.py", "-m", "gpu", "-k",
+                 "server or batch or budget or noalloc or autotune or graph or alone or "
+                 "eval or trajectory_small_populations or main_errors"],
                 env, timeout=900)
     assert r.returncode == 0
     assert " passed" in r.stdout and "AddressSanitizer" not in r.stderr
