@@ -351,13 +351,19 @@ def bench_interest_calib(reps=5, cpu=True):
     from locvolcalib import futhark_values as fv
     from locvolcalib import interestcalib as icl
     data = os.path.join(ROOT, "tests", "golden", "InterestCalib-data")
+    # VALU instructions of one ic_eval_kernel launch on small (rocprofv3 PMC,
+    # the profile beside): the issue fraction below divides it by the launch's
+    # live duration
+    counters = json.load(open(os.path.join(ROOT, "profiles",
+                                           "ic_eval_kernel_interestcalib_small.counters.json")))
     res = {}
-    with icl.InterestCalib(device=0) as ctx:
+    with icl.InterestCalib(device=0, profiling=True) as ctx:
         for name in ("small", "medium", "large"):
             a = icl.main_arguments(fv.load(os.path.join(data, f"{name}.in")))
             expected = float(fv.load(os.path.join(data, f"{name}.out"))[5])
             hs = ctx.upload_inputs(a["swaptions"], a["herm_x"], a["herm_w"], a["dir_vct"])
             try:
+                ctx.pause_profiling()
                 out, h = ctx.main_raw(a["pop"], a["mcmc_conv"], hs)
                 ctx.free_summary(h)
                 t0 = time.perf_counter()
@@ -365,15 +371,44 @@ def bench_interest_calib(reps=5, cpu=True):
                     out, h = ctx.main_raw(a["pop"], a["mcmc_conv"], hs)
                     ctx.free_summary(h)
                 dt = (time.perf_counter() - t0) / reps
+                # one more call with events around every launch (the report
+                # accumulates: this call's share is the difference)
+                rep0 = json.loads(ctx.report())
+                ctx.unpause_profiling()
+                _, h = ctx.main_raw(a["pop"], a["mcmc_conv"], hs)
+                ctx.free_summary(h)
+                rep1 = json.loads(ctx.report())
+                ctx.pause_profiling()
+                zero = {"count": 0, "total_us": 0.0}
+                rep = {k: {"count": v["count"] - rep0.get(k, zero)["count"],
+                           "total_us": v["total_us"] - rep0.get(k, zero)["total_us"]}
+                       for k, v in rep1.items() if isinstance(v, dict)}
             finally:
                 ctx.free_inputs(hs)
             nsw = len(a["swaptions"])
             evals = a["pop"] * nsw * (a["mcmc_conv"] + 1)
+            gpu_us = sum(v["total_us"] for v in rep.values())
+            ev = rep.get("ic_eval", {"count": 0, "total_us": 0.0})
+            # the eval launches' share of the event-timed GPU time, applied to
+            # the event-free call time (the launches run back to back)
+            eval_us = (dt * 1e6 * ev["total_us"] / gpu_us / max(ev["count"], 1)
+                       if gpu_us > 0 else 0.0)
             r = {"pop": a["pop"], "mcmc_conv": a["mcmc_conv"], "swaptions": nsw,
                  "ms_per_call": round(dt * 1e3, 3),
                  "iterations_per_s": round(a["mcmc_conv"] / dt, 1),
                  "evaluations_per_s": round(evals / dt, 1),
-                 "winner_loglik": float(out[5]), "expected_loglik": expected}
+                 "winner_loglik": float(out[5]), "expected_loglik": expected,
+                 "kernels_us_with_events": {k: round(v["total_us"] / max(v["count"], 1), 3)
+                                            for k, v in rep.items()},
+                 "eval_kernel_us": round(eval_us, 3)}
+            if name == "small" and eval_us > 0:
+                peak = 1024 * 2.4e9 / 4  # wave64 VALU instructions per second
+                achieved = counters["SQ_INSTS_VALU_per_launch"] / (eval_us * 1e-6)
+                r["roofline"] = {"bound": "valu", "kernel": "ic_eval_kernel",
+                                 "achieved": round(achieved, 1), "peak": peak,
+                                 "unit": "wave64 VALU instructions/s",
+                                 "frac": round(achieved / peak, 4),
+                                 "how": counters["how"]}
             if cpu:
                 from oracle import oracle
                 g = np.array([[np.float32(oracle.ic_sobol(5 * i + 1 + j, a["dir_vct"]))
@@ -515,8 +550,9 @@ def main():
         total32, lat32 = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps,
                                     args.warmup)
         # the same calls again with HIP events around every launch on the
-        # library's stream: per-kernel durations for the roofline
-        ctx.report()  # drain
+        # library's stream: per-kernel durations for the roofline (the
+        # report accumulates; nothing was recorded while paused, so the next
+        # report holds these calls alone)
         ctx.unpause_profiling()
         total32p, _ = time_calls(ctx, ctx.main_raw, ctx.free_f32, args.steps, 0)
         rep32 = json.loads(ctx.report())

@@ -296,21 +296,37 @@ static int run_calibration(struct futhark_context *ctx, ic_call *c, int32_t conv
   float *logliks = props + 5 * (size_t)pop, *fb = logliks + pop;
   float *out = fb + pop, *winner = out + 8;
   const ic_move none = {0, 0, 0, 0};
-  int tok = lvc_prof_begin(ctx, "ic_calibration");
+  /* with profiling on, each launch is timed under its name (events on the
+   * context's stream): ic_eval (the population's evaluations), ic_step
+   * (accept + propose), ic_other (init, winner, summary) */
+  int tok = lvc_prof_begin(ctx, "ic_other");
   int e = ic_launch_init(st, pop, dv, genomes, props);
-  if (!e)
+  lvc_prof_end(ctx, tok);
+  if (!e) {
+    tok = lvc_prof_begin(ctx, "ic_eval");
     e = ic_launch_eval(st, c->sw, c->per, &c->pop_lists, genomes, pop, hx, hw, c->nherm,
                        c->nswap, IC_OUT_TERMS, c->terms, NULL);
-  if (!e)
+    lvc_prof_end(ctx, tok);
+  }
+  if (!e) {
+    tok = lvc_prof_begin(ctx, "ic_step");
     e = ic_launch_step(st, pop, c->nswap, c->terms, genomes, props, logliks, fb, 1, 0,
                        conv > 0 ? &moves[0] : &none, dv, -1, conv, trace);
+    lvc_prof_end(ctx, tok);
+  }
   for (int32_t it = 0; !e && it < conv; it++) {
+    tok = lvc_prof_begin(ctx, "ic_eval");
     e = ic_launch_eval(st, c->sw, c->per, &c->pop_lists, props, pop, hx, hw, c->nherm,
                        c->nswap, IC_OUT_TERMS, c->terms, NULL);
-    if (!e)
+    lvc_prof_end(ctx, tok);
+    if (!e) {
+      tok = lvc_prof_begin(ctx, "ic_step");
       e = ic_launch_step(st, pop, c->nswap, c->terms, genomes, props, logliks, fb, 0, acc[it],
                          it + 1 < conv ? &moves[it + 1] : &none, dv, it, conv, trace);
+      lvc_prof_end(ctx, tok);
+    }
   }
+  tok = lvc_prof_begin(ctx, "ic_other");
   if (!e) e = ic_launch_winner(st, pop, genomes, logliks, out, winner);
   if (!e && c->nswap > 0)
     e = ic_launch_eval(st, c->sw, c->per, &c->one_lists, winner, 1, hx, hw, c->nherm, c->nswap,

@@ -194,7 +194,16 @@ class InterestCalib:
         self._check(self._L.futhark_context_sync(self._ctx), "futhark_context_sync")
 
     def report(self) -> str:
+        """``futhark_context_report``: with profiling on, the GPU time of
+        each kind of launch since the last report (ic_eval, ic_step,
+        ic_other), as JSON."""
         return _take_string(self._L.futhark_context_report(self._ctx)) or "{}"
+
+    def pause_profiling(self) -> None:
+        self._L.futhark_context_pause_profiling(self._ctx)
+
+    def unpause_profiling(self) -> None:
+        self._L.futhark_context_unpause_profiling(self._ctx)
 
     # -- arrays -------------------------------------------------------------
     def _inputs(self, swaptions, herm_x, herm_w, dir_vct=None):
