@@ -218,13 +218,14 @@ def test_trajectory_small_dataset(ic):
     print(f"  decisions within the expf difference of their draw: {flips}", flush=True)
 
 
-def test_trajectory_reduced_medium(ic):
-    print("case: interestcalib_trace, medium (pop 128, 60 iterations, 98 swaptions)",
+@pytest.mark.parametrize("name", ["medium", "large"])
+def test_trajectory_full_datasets(ic, name):
+    print(f"case: interestcalib_trace, {name} (pop 128, 424 iterations, 196 swaptions)",
           flush=True)
-    pop, conv, sw, hx, hw, dv = trajectory_case("medium", conv=60)
-    sw = sw[::2]
+    pop, conv, sw, hx, hw, dv = trajectory_case(name)
     tr = ic.trace(pop, conv, sw, hx, hw, dv)
-    check_trajectory(tr, pop, conv, sw, hx, hw, dv, ll_iters=set(range(0, conv, 10)))
+    flips = check_trajectory(tr, pop, conv, sw, hx, hw, dv, ll_iters=set(range(0, conv, 40)))
+    print(f"  decisions within the expf difference of their draw: {flips}", flush=True)
 
 
 @pytest.mark.parametrize("pop, conv", [(1, 3), (2, 12), (3, 9), (70, 5)])
