@@ -284,3 +284,17 @@ def test_main_errors(ic):
         ic.main(pop, conv, sw, hx, hw, np.ones(32, np.int32))
     with pytest.raises(InterestCalibError, match="negative"):
         ic.main(pop, conv, np.array([[1, 6, -1, 0.2]], np.float32), hx, hw, dv)
+
+
+def test_main_without_swaptions_is_bit_exact(ic):
+    """With no swaption every log-likelihood is 0 (an empty sum, :641-645),
+    so the accept step compares min(1, exp(0) * fb) with its draw: the
+    whole trajectory is f32 +-*/ min max and Sobol numbers, and the GPU's
+    winner must be the oracle's bit for bit."""
+    print("case: main with no swaptions (pop 12, 40 iterations)", flush=True)
+    _, _, _, hx, hw, dv = trajectory_case("medium")
+    sw = np.zeros((0, 4), np.float32)
+    out, summ = ic.main(12, 40, sw, hx, hw, dv)
+    want, wsumm, _, _ = oracle.ic_main(12, 40, sw, hx, hw, dv)
+    assert summ.shape == (0, 3) and wsumm.shape == (0, 3)
+    assert np.array_equal(out, want), (out, want)
