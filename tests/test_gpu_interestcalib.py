@@ -298,3 +298,20 @@ def test_main_without_swaptions_is_bit_exact(ic):
     want, wsumm, _, _ = oracle.ic_main(12, 40, sw, hx, hw, dv)
     assert summ.shape == (0, 3) and wsumm.shape == (0, 3)
     assert np.array_equal(out, want), (out, want)
+
+
+def test_main_without_hermite_points(ic):
+    """No quadrature points: every model price is 0 (an empty sum, :594-620),
+    so every genome has the same log-likelihood and the accept step again
+    compares min(1, fb) with its draw; the winner's genome equals the
+    oracle's bit for bit and its log-likelihood (a sum of Cauchy terms of
+    the quotes) agrees within 1e-6 relative."""
+    print("case: main with no Hermite points (pop 16, 30 iterations)", flush=True)
+    _, _, sw, _, _, dv = trajectory_case("medium")
+    sw = sw[::14]
+    e = np.zeros(0, np.float32)
+    out, summ = ic.main(16, 30, sw, e, e, dv)
+    want, wsumm, _, _ = oracle.ic_main(16, 30, sw, e, e, dv)
+    assert np.array_equal(out[:5], want[:5]), (out, want)
+    assert abs(float(out[5]) - float(want[5])) <= 1e-6 * abs(float(want[5]))
+    assert np.all(summ[:, 0] == 0.0) and np.all(wsumm[:, 0] == 0.0)
