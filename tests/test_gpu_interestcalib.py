@@ -130,14 +130,16 @@ def test_eval_at_the_medium_winner(ic):
 
 
 def test_eval_synthetic_schedules(ic):
-    """One-period swaptions (1 lane), 120 and 360 periods (the long-schedule
-    kernel, 2 and 6 periods a lane), a zero-period schedule (NaN, as the
-    oracle), against the oracle."""
+    """One-period swaptions (1 period a lane), 120 periods (2 a lane over 64
+    lanes), 204 and 240 (4 a lane) and 360 (8 a lane, the long-schedule
+    kernel), 64, a zero-period schedule (NaN, as the oracle), against the
+    oracle."""
     print("case: interestcalib_eval, synthetic schedules", flush=True)
     args, _, _ = data("small")
     hx, hw = args["herm_x"], args["herm_w"]
     sw = np.array([[1, 12, 1, 0.3], [5, 12, 1, 0.3], [2, 1, 10, 0.2], [1, 1, 30, 0.2],
-                   [3, 3, 16, 0.25], [2, 6, 32, 0.2], [4, 12, 0, 0.3]], np.float32)
+                   [3, 3, 16, 0.25], [2, 6, 32, 0.2], [1, 1, 17, 0.2], [1, 2, 40, 0.2],
+                   [4, 12, 0, 0.3]], np.float32)
     g = well_conditioned(70, 2)  # more genomes than one wavefront holds at every width
     q, p, ll = ic.eval_genomes(g, sw, hx, hw)
     oq, op = oracle_eval(g, sw, hx, hw)
@@ -315,3 +317,18 @@ def test_main_without_hermite_points(ic):
     assert np.array_equal(out[:5], want[:5]), (out, want)
     assert abs(float(out[5]) - float(want[5])) <= 1e-6 * abs(float(want[5]))
     assert np.all(summ[:, 0] == 0.0) and np.all(wsumm[:, 0] == 0.0)
+
+
+def test_trajectory_random_direction_vector(ic):
+    """A random 30-bit direction vector (every Sobol number the XOR of a
+    different subset) and random swaptions of 1..144 periods: every
+    iteration pinned as on the datasets."""
+    print("case: interestcalib_trace, random direction vector (pop 40, 30 iterations)",
+          flush=True)
+    rng = np.random.default_rng(11)
+    dv = rng.integers(1, 1 << 30, 30).astype(np.int32)
+    _, _, _, hx, hw, _ = trajectory_case("small")
+    sw = np.stack([rng.integers(1, 11, 24), rng.choice([1, 3, 6, 12], 24),
+                   rng.integers(1, 13, 24), rng.uniform(0.15, 0.6, 24)], axis=1).astype(np.float32)
+    tr = ic.trace(40, 30, sw, hx, hw, dv)
+    check_trajectory(tr, 40, 30, sw, hx, hw, dv, ll_iters=set(range(0, 30, 5)))
