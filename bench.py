@@ -521,6 +521,8 @@ def main():
                     help="Poisson arrival rates (req/s) of the serving runs")
     ap.add_argument("--only-fp32refine", action="store_true",
                     help="run only the FP32+refinement calls (for profiling)")
+    ap.add_argument("--only-ocean", action="store_true",
+                    help="run only the ocean-sim tridiagonal entries (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB (read) + 256 MiB "
                          "(write) three times, to scale HBM traffic counters")
@@ -543,6 +545,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+
+    if args.only_ocean:
+        print(json.dumps({"ocean_tridiag": bench_ocean(lvc)}), flush=True)
+        return
 
     with lvc.LocVolCalib(device=0, profiling=True) as ctx:
         # FP32 + refinement (headline): timed calls without per-launch events

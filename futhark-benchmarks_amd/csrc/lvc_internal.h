@@ -160,8 +160,8 @@ int lvc_launch_setup_list(void *stream, int precision, lvc_problem *probs,
 
 /* Batched f32 tridiagonal solves of [n][m] systems (the ocean-sim entries):
  * form 0 = the scan form (tridagPar), 1 = Thomas (tridagSeq).  scratch holds
- * n*m floats and is needed when m <= lvc_tridag_staged_rows() or
- * m > lvc_tridag_lds_rows(). */
+ * n*m floats and is needed when m > lvc_tridag_lds_rows(); systems of up to
+ * lvc_tridag_staged_rows() unknowns keep their forward values in registers. */
 int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
                       const float *c, const float *y, float *out, float *scratch,
                       int64_t n, int m);

@@ -64,8 +64,7 @@ static int tridag_entry(struct futhark_context *ctx, struct futhark_f32_2d **out
   out->a.bytes = (size_t)n * (size_t)m * sizeof(float);
   pthread_mutex_lock(&ctx->lock);
   void *scratch = NULL;
-  const size_t sbytes =
-      (m <= lvc_tridag_staged_rows() || m > lvc_tridag_lds_rows()) ? out->a.bytes : 0;
+  const size_t sbytes = m > lvc_tridag_lds_rows() ? out->a.bytes : 0;
   int rc = lvc_dev_alloc(ctx, out->a.bytes, &out->a.mem) ||
            (sbytes && lvc_dev_alloc(ctx, sbytes, &scratch));
   if (!rc && n > 0) {

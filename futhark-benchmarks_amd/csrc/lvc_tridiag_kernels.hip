@@ -7,15 +7,16 @@
 //                      in the form of the reference's scans, evaluated left to
 //                      right.
 //
-// One thread per system (systems of up to 128 unknowns through the staged
-// kernel below), in the reference's order of operations with IEEE
+// One thread per system, in the reference's order of operations with IEEE
 // division and no contraction (the file is built with -ffp-contract=off), so
 // the results are the bits of a sequential f32 evaluation: these systems are
 // not diagonally dominant, and a reordered (partitioned or cyclic-reduction)
-// solve would meet pivots the reference's never does.  The forward pass keeps
-// its per-row values for the backward pass in LDS (c' or the modified
-// diagonal, one column per lane) when m <= kLdsRows, otherwise in a global
-// scratch array; y' goes to the output array in place.
+// solve would meet pivots the reference's never does.  Systems of up to 128
+// unknowns go through the staged kernel below, which keeps the forward
+// pass's per-row values in registers.  For longer ones the two kernels above
+// keep them in LDS (c' or the modified diagonal, one column per lane) when
+// m <= kLdsRows, otherwise in a global scratch array; y' goes to the output
+// array in place.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -128,189 +129,256 @@ __global__ __launch_bounds__(kThreads) void tridag_par_kernel(
 }
 
 // Staged variant for systems of up to kStagedRows unknowns: one wavefront per
-// 64 systems.  Chunks of kChunk columns of a, b, c, y are loaded into LDS
-// with coalesced row reads (each system's row is contiguous; kChunk * 4 =
-// 128 bytes per system row) and read back one system per lane.  The forward
-// pass's per-row values for the backward pass go back through the same
-// transpose to global memory: y' (or l0) into the output array in place, c'
-// (or the scan form's l1) into a scratch array of the same shape, so that
-// the workgroup's LDS is the staging buffer alone and every workgroup of a
-// launch is resident at once.  The backward pass reads both back chunk by
-// chunk in reverse.  The arithmetic and its order per system are those of
-// the kernels above (recurrences 1-3 of the scan form advance together: each
-// step uses only earlier steps' values, so interleaving them changes no
-// result).
+// 64 systems.  Chunks of CH columns of a, b, c, y are loaded into LDS with
+// coalesced row reads (each system's row is contiguous: CH * 4 bytes of a
+// row per chunk, 16 bytes a lane) and read back one system per lane.  The forward
+// pass's two per-row values for the backward pass (c' and y' of Thomas, l1
+// and l0 of the scan form's third recurrence) stay in this lane's registers:
+// every loop over rows is unrolled to kStagedRows steps, so each row has a
+// register of its own, and with one wavefront per SIMD (launches of this
+// kernel never hold more) the 2 x kStagedRows of them fit in the unified
+// register file.  The backward pass therefore reads nothing from memory; its
+// solution goes out through the same LDS transpose, coalesced.  The
+// arithmetic and its order per system are those of the kernels above
+// (recurrences 1-3 of the scan form advance together: each step uses only
+// earlier steps' values, so interleaving them changes no result).
+//
+// Chunk width: Thomas takes 32 columns (128-byte row reads); the scan form,
+// with more carries and five divisions a row, takes 16, the widest whose
+// in-flight next chunk (4 x CH values a lane) fits beside the 256 row
+// registers without spilling.
 constexpr int kStagedRows = 128;
-constexpr int kChunk = 32;
-constexpr int kRowStep = kThreads / kChunk;  // rows per staging instruction
-constexpr int kQ = kThreads / kRowStep;      // staging instructions per array
+constexpr int kVec = 4;  // floats per global load and store of the staging
+// dword-aligned float4 (a row of m floats starts at any multiple of 4 bytes)
+typedef float float4u __attribute__((ext_vector_type(kVec), aligned(4)));
 
+// A chunk of one array, CH columns of the 64 rows, is moved in kQ
+// instructions of 16 bytes a lane: kLanesPerRow lanes cover one row's CH
+// columns, kRowStep rows per instruction.  Whole instructions, not bytes,
+// are what bound a single wavefront here: gfx9 tracks at most 63 vector
+// memory instructions in flight per wave, so 4-float moves quarter the
+// round trips of 1-float ones.
+template <int CH>
 struct Stage {
-  float v[4][kThreads][kChunk + 1];  // a, b, c, y; after use c' (l1) and y' (l0)
+  static constexpr int kLanesPerRow = CH / kVec;
+  static constexpr int kRowStep = kThreads / kLanesPerRow;
+  static constexpr int kQ = kThreads / kRowStep;
+  float v[4][kThreads][CH + 1];  // a, b, c, y; the solution through v[3]
 };
 
 // The same rows and columns loaded into registers (this lane's slots), to be
 // written to LDS later: the next chunk's loads fly while this one computes.
-__device__ __forceinline__ void fetch(float (&reg)[kQ], const float *__restrict__ src,
-                                      int64_t s0, int rows, int m, int i0, int lane) {
-  const int sr = lane / kChunk, sc = lane % kChunk, i = i0 + sc;
+// Slots outside the systems (rows >= rows, columns >= m) hold 0.  A chunk
+// wholly inside them (every chunk of a full workgroup but a ragged last one)
+// takes unguarded 16-byte loads; the test is made once for the four arrays,
+// so that their loads issue back to back.
+template <int CH>
+using Chunk = float[Stage<CH>::kQ][kVec];
+
+template <int CH>
+__device__ __forceinline__ void fetch(Chunk<CH> &pa, Chunk<CH> &pb, Chunk<CH> &pc,
+                                      Chunk<CH> &py, const float *__restrict__ a,
+                                      const float *__restrict__ b, const float *__restrict__ c,
+                                      const float *__restrict__ y, int64_t s0, int rows, int m,
+                                      int i0, int lane) {
+  using St = Stage<CH>;
+  const int sr = lane / St::kLanesPerRow, sc = (lane % St::kLanesPerRow) * kVec;
+  const int i = i0 + sc;
+  const float *__restrict__ src[4] = {a, b, c, y};
+  float(*const dst[4])[kVec] = {pa, pb, pc, py};
+  if (rows == kThreads && i0 + CH <= m) {
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) {
-    const int r = q * kRowStep + sr;
-    reg[q] = (r < rows && i < m) ? src[(s0 + r) * m + i] : 0.0f;
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int q = 0; q < St::kQ; ++q) {
+        const float4u v = *(const float4u *)(src[k] + (s0 + q * St::kRowStep + sr) * m + i);
+        dst[k][q][0] = v.x;
+        dst[k][q][1] = v.y;
+        dst[k][q][2] = v.z;
+        dst[k][q][3] = v.w;
+      }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int q = 0; q < St::kQ; ++q) {
+        const int r = q * St::kRowStep + sr;
+#pragma unroll
+        for (int e = 0; e < kVec; ++e)
+          dst[k][q][e] = (r < rows && i + e < m) ? src[k][(s0 + r) * m + i + e] : 0.0f;
+      }
   }
 }
 
-__device__ __forceinline__ void deposit(float (&st)[kThreads][kChunk + 1],
-                                        const float (&reg)[kQ], int lane) {
-  const int sr = lane / kChunk, sc = lane % kChunk;
+template <int CH>
+__device__ __forceinline__ void deposit(float (&st)[kThreads][CH + 1], const Chunk<CH> &reg,
+                                        int lane) {
+  using St = Stage<CH>;
+  const int sr = lane / St::kLanesPerRow, sc = (lane % St::kLanesPerRow) * kVec;
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) st[q * kRowStep + sr][sc] = reg[q];
+  for (int q = 0; q < St::kQ; ++q)
+#pragma unroll
+    for (int e = 0; e < kVec; ++e) st[q * St::kRowStep + sr][sc + e] = reg[q][e];
 }
 
-__device__ __forceinline__ void stage_out(const float (&st)[kThreads][kChunk + 1],
+template <int CH>
+__device__ __forceinline__ void stage_out(const float (&st)[kThreads][CH + 1],
                                           float *__restrict__ dst, int64_t s0, int rows,
                                           int m, int i0, int lane) {
-  const int sr = lane / kChunk, sc = lane % kChunk, i = i0 + sc;
+  using St = Stage<CH>;
+  const int sr = lane / St::kLanesPerRow, sc = (lane % St::kLanesPerRow) * kVec;
+  const int i = i0 + sc;
+  float v[St::kQ][kVec];
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) {
-    const int r = q * kRowStep + sr;
-    if (r < rows && i < m) dst[(s0 + r) * m + i] = st[r][sc];
+  for (int q = 0; q < St::kQ; ++q)
+#pragma unroll
+    for (int e = 0; e < kVec; ++e) v[q][e] = st[q * St::kRowStep + sr][sc + e];
+  if (rows == kThreads && i0 + CH <= m) {
+#pragma unroll
+    for (int q = 0; q < St::kQ; ++q) {
+      float4u w;
+      w.x = v[q][0];
+      w.y = v[q][1];
+      w.z = v[q][2];
+      w.w = v[q][3];
+      *(float4u *)(dst + (s0 + q * St::kRowStep + sr) * m + i) = w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < St::kQ; ++q) {
+      const int r = q * St::kRowStep + sr;
+#pragma unroll
+      for (int e = 0; e < kVec; ++e)
+        if (r < rows && i + e < m) dst[(s0 + r) * m + i + e] = v[q][e];
+    }
   }
 }
 
-template <bool SEQ>
-__global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
-    const float *__restrict__ a, const float *__restrict__ b,
-    const float *__restrict__ c, const float *__restrict__ y,
-    float *__restrict__ out, float *__restrict__ cw, int64_t n, int m) {
-  __shared__ Stage S;
+template <bool SEQ, int CH>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void tridag_staged_kernel(const float *__restrict__ a, const float *__restrict__ b,
+                          const float *__restrict__ c, const float *__restrict__ y,
+                          float *__restrict__ out, int64_t n, int m) {
+  constexpr int kChunks = kStagedRows / CH;
+  constexpr int kQ = Stage<CH>::kQ;
+  __shared__ Stage<CH> S;
   const int lane = threadIdx.x;
   const int64_t s0 = (int64_t)blockIdx.x * kThreads;
   const int rows = (int)min((int64_t)kThreads, n - s0);
-  const int nch = (m + kChunk - 1) / kChunk;
+  const int nch = (m + CH - 1) / CH;
+  float fw0[kStagedRows], fw1[kStagedRows];  // y' and c' (scan form: l0, l1)
   // Thomas carries
   float cprev = 0.0f, yprev = 0.0f;
   // scan-form carries
   float t0 = 1.0f, t1 = 0.0f, t2 = 0.0f, t3 = 1.0f, b0 = 0.0f, bprev = 0.0f;
-  float f0 = 0.0f, f1 = 1.0f, y0 = 0.0f, last_bb = 1.0f, last_yy = 0.0f;
+  float f0 = 0.0f, f1 = 1.0f, y0 = 0.0f;
+  float yn = 0.0f;  // scan form: l0 of row m-1, recurrence 3's start
 
-  float pa[kQ], pb[kQ], pc[kQ], py[kQ];  // the next chunk, in flight
-  fetch(pa, a, s0, rows, m, 0, lane);
-  fetch(pb, b, s0, rows, m, 0, lane);
-  fetch(pc, c, s0, rows, m, 0, lane);
-  fetch(py, y, s0, rows, m, 0, lane);
-  for (int ch = 0; ch < nch; ++ch) {
-    const int i0 = ch * kChunk;
-    deposit(S.v[0], pa, lane);
-    deposit(S.v[1], pb, lane);
-    deposit(S.v[2], pc, lane);
-    deposit(S.v[3], py, lane);
+  // The row loops have no per-row branch, so that the scheduler can overlap
+  // one row's LDS reads with the previous row's arithmetic: rows m..(the end
+  // of the last chunk) run on the zeros that fetch() leaves there, and what
+  // they produce is never stored nor carried into rows below m (the forward
+  // pass ends with them; the backward pass restarts at row m-1 by select).
+  Chunk<CH> pa, pb, pc, py;  // the next chunk, in flight
+  fetch<CH>(pa, pb, pc, py, a, b, c, y, s0, rows, m, 0, lane);
+#pragma unroll
+  for (int ch = 0; ch < kChunks; ++ch) {
+    if (ch >= nch) break;
+    const int i0 = ch * CH;
+    deposit<CH>(S.v[0], pa, lane);
+    deposit<CH>(S.v[1], pb, lane);
+    deposit<CH>(S.v[2], pc, lane);
+    deposit<CH>(S.v[3], py, lane);
     __syncthreads();
     if (ch + 1 < nch) {
-      fetch(pa, a, s0, rows, m, i0 + kChunk, lane);
-      fetch(pb, b, s0, rows, m, i0 + kChunk, lane);
-      fetch(pc, c, s0, rows, m, i0 + kChunk, lane);
-      fetch(py, y, s0, rows, m, i0 + kChunk, lane);
+      fetch<CH>(pa, pb, pc, py, a, b, c, y, s0, rows, m, i0 + CH, lane);
     }
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
+    for (int j = 0; j < CH; ++j) {
       const int i = i0 + j;
-      if (i < m) {
-        const float ai = S.v[0][lane][j], bi = S.v[1][lane][j];
-        const float ci = S.v[2][lane][j], yi = S.v[3][lane][j];
-        if (SEQ) {
-          if (i == 0) {
-            cprev = ci / bi;
-            yprev = yi / bi;
-          } else {
-            const float norm = 1.0f / (bi - ai * cprev);
-            cprev = ci * norm;
-            yprev = (yi - ai * yprev) * norm;
-          }
-          S.v[2][lane][j] = cprev;
-          S.v[3][lane][j] = yprev;
+      const float ai = S.v[0][lane][j], bi = S.v[1][lane][j];
+      const float ci = S.v[2][lane][j], yi = S.v[3][lane][j];
+      if (SEQ) {
+        if (i == 0) {
+          cprev = ci / bi;
+          yprev = yi / bi;
         } else {
-          // recurrence 1 (c_{i-1} carried in cprev)
-          if (i == 0) {
-            b0 = bi;
-          } else {
-            const float m0 = bi, m1 = 0.0f - ai * cprev, m2 = 1.0f, m3 = 0.0f;
-            const float value = 1.0f / (t0 * m0);
-            const float n0 = (m0 * t0 + m1 * t2) * value;
-            const float n1 = (m0 * t1 + m1 * t3) * value;
-            const float n2 = (m2 * t0 + m3 * t2) * value;
-            const float n3 = (m2 * t1 + m3 * t3) * value;
-            t0 = n0;
-            t1 = n1;
-            t2 = n2;
-            t3 = n3;
-          }
-          const float bb = (t0 * b0 + t1) / (t2 * b0 + t3);
-          // recurrence 2
-          float yy;
-          if (i == 0) {
-            y0 = yi;
-            yy = f0 + f1 * y0;
-          } else {
-            const float l0 = yi, l1 = 0.0f - ai / bprev;
-            const float g0 = l0 + l1 * f0, g1 = f1 * l1;
-            f0 = g0;
-            f1 = g1;
-            yy = f0 + f1 * y0;
-          }
-          // this row's element of recurrence 3
-          S.v[3][lane][j] = yy / bb;
-          S.v[2][lane][j] = 0.0f - ci / bb;
-          bprev = bb;
-          cprev = ci;
-          last_bb = bb;
-          last_yy = yy;
+          const float norm = 1.0f / (bi - ai * cprev);
+          cprev = ci * norm;
+          yprev = (yi - ai * yprev) * norm;
         }
+        fw1[i] = cprev;
+        fw0[i] = yprev;
+      } else {
+        // recurrence 1 (c_{i-1} carried in cprev)
+        if (i == 0) {
+          b0 = bi;
+        } else {
+          const float m0 = bi, m1 = 0.0f - ai * cprev, m2 = 1.0f, m3 = 0.0f;
+          const float value = 1.0f / (t0 * m0);
+          const float n0 = (m0 * t0 + m1 * t2) * value;
+          const float n1 = (m0 * t1 + m1 * t3) * value;
+          const float n2 = (m2 * t0 + m3 * t2) * value;
+          const float n3 = (m2 * t1 + m3 * t3) * value;
+          t0 = n0;
+          t1 = n1;
+          t2 = n2;
+          t3 = n3;
+        }
+        const float bb = (t0 * b0 + t1) / (t2 * b0 + t3);
+        // recurrence 2
+        float yy;
+        if (i == 0) {
+          y0 = yi;
+          yy = f0 + f1 * y0;
+        } else {
+          const float l0 = yi, l1 = 0.0f - ai / bprev;
+          const float g0 = l0 + l1 * f0, g1 = f1 * l1;
+          f0 = g0;
+          f1 = g1;
+          yy = f0 + f1 * y0;
+        }
+        // this row's element of recurrence 3
+        fw0[i] = yy / bb;
+        fw1[i] = 0.0f - ci / bb;
+        bprev = bb;
+        cprev = ci;
+        yn = i < m ? fw0[i] : yn;  // ends as y'_{m-1}/b'_{m-1}
       }
     }
-    __syncthreads();
-    stage_out(S.v[2], cw, s0, rows, m, i0, lane);
-    stage_out(S.v[3], out, s0, rows, m, i0, lane);
-    __syncthreads();
+    __syncthreads();  // every lane has read the chunk before the next deposit
   }
 
-  // backward pass over the chunks in reverse; the solution replaces y' / l0
-  float xnext = yprev;                                  // Thomas: x_{m-1} = y'_{m-1}
-  const float yn = SEQ ? 0.0f : last_yy / last_bb;      // scan form
-  float g0 = 0.0f, g1 = 1.0f;
-  fetch(pc, cw, s0, rows, m, (nch - 1) * kChunk, lane);
-  fetch(py, out, s0, rows, m, (nch - 1) * kChunk, lane);
-  for (int ch = nch - 1; ch >= 0; --ch) {
-    const int i0 = ch * kChunk;
-    deposit(S.v[2], pc, lane);
-    deposit(S.v[3], py, lane);
-    __syncthreads();
-    if (ch > 0) {
-      fetch(pc, cw, s0, rows, m, i0 - kChunk, lane);
-      fetch(py, out, s0, rows, m, i0 - kChunk, lane);
-    }
+  // backward pass over the chunks in reverse, from registers
+  float xnext = 0.0f;         // Thomas
+  float g0 = 0.0f, g1 = 1.0f;  // scan form
 #pragma unroll
-    for (int j = kChunk - 1; j >= 0; --j) {
-      const int i = i0 + j;
-      if (i < m) {
-        const float fwi = S.v[2][lane][j];
-        if (i == m - 1) {
-          S.v[3][lane][j] = SEQ ? yprev : 0.0f + 1.0f * yn;
-        } else if (SEQ) {
-          xnext = S.v[3][lane][j] - fwi * xnext;
-          S.v[3][lane][j] = xnext;
+  for (int ch = kChunks - 1; ch >= 0; --ch) {
+    if (ch < nch) {
+      const int i0 = ch * CH;
+#pragma unroll
+      for (int j = CH - 1; j >= 0; --j) {
+        const int i = i0 + j;
+        float x;
+        if (SEQ) {
+          // x_{m-1} = y'_{m-1}; below it x_i = y'_i - c'_i x_{i+1}
+          x = fw0[i] - fw1[i] * xnext;
+          x = i == m - 1 ? fw0[i] : x;
+          xnext = x;
         } else {
-          const float h0 = S.v[3][lane][j] + fwi * g0, h1 = g1 * fwi;
-          g0 = h0;
-          g1 = h1;
-          S.v[3][lane][j] = g0 + g1 * yn;
+          // recurrence 3 restarts at row m-1 from (g0, g1) = (0, 1)
+          const float h0 = fw0[i] + fw1[i] * g0, h1 = g1 * fw1[i];
+          const bool below = i < m - 1;
+          g0 = below ? h0 : 0.0f;
+          g1 = below ? h1 : 1.0f;
+          x = g0 + g1 * yn;
         }
+        S.v[3][lane][j] = x;
       }
+      __syncthreads();
+      stage_out<CH>(S.v[3], out, s0, rows, m, i0, lane);
+      __syncthreads();
     }
-    __syncthreads();
-    stage_out(S.v[3], out, s0, rows, m, i0, lane);
-    __syncthreads();
   }
 }
 
@@ -319,8 +387,8 @@ __global__ __launch_bounds__(kThreads) void tridag_staged_kernel(
 extern "C" {
 
 // form: 0 = tridagPar (scan form), 1 = tridagSeq (Thomas).  scratch: n*m
-// floats, used when m <= lvc_tridag_staged_rows() (the staged kernel's c')
-// or m > lvc_tridag_lds_rows(); may be NULL otherwise.
+// floats, used when m > lvc_tridag_lds_rows() (the forward pass's per-row
+// values); may be NULL otherwise.
 int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
                       const float *c, const float *y, float *out, float *scratch,
                       int64_t n, int m) {
@@ -331,11 +399,10 @@ int lvc_launch_tridag(void *stream, int form, const float *a, const float *b,
   if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
   if (form != 0 && form != 1) return (int)hipErrorInvalidValue;
   if (m <= kStagedRows) {
-    if (!scratch) return (int)hipErrorInvalidValue;
     if (form == 1)
-      tridag_staged_kernel<true><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, scratch, n, m);
+      tridag_staged_kernel<true, 32><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, n, m);
     else
-      tridag_staged_kernel<false><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, scratch, n, m);
+      tridag_staged_kernel<false, 16><<<(unsigned)blocks, kThreads, 0, st>>>(a, b, c, y, out, n, m);
     return (int)hipGetLastError();
   }
   const bool lds = m <= kLdsRows;

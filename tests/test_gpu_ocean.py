@@ -6,8 +6,9 @@ golden output tridiag32-small.out; tridagNested(Const) (the scan form)
 bit-exact against the oracle's restatement of the same lines
 (oracle.tridag_par4) and within the oracle's own loose agreement with the
 golden output (these random systems are not diagonally dominant).  Larger
-random inputs (the reference's `random input` shape class, and systems longer
-than the LDS-resident limit) bit-exact against the oracle."""
+random inputs (the reference's `random input` shape class, the edges of the
+register-resident kernel's chunks and of its 128-unknown limit, and systems
+longer than the LDS-resident limit) bit-exact against the oracle."""
 import os
 
 import numpy as np
@@ -63,7 +64,11 @@ def test_par_bit_exact_vs_oracle(ctx, golden, const):
     assert np.median(rel) < 1e-5 and np.percentile(rel, 99) < 1e-2
 
 
-@pytest.mark.parametrize("n,m", [(3000, 115), (70, 300), (65, 1), (64, 2), (129, 256), (5, 257)])
+# m around the staged kernel's chunk edges (16 and 32 columns) and its
+# 128-unknown limit, n around its 64-system workgroups
+@pytest.mark.parametrize("n,m", [(3000, 115), (70, 300), (65, 1), (64, 2), (129, 256), (5, 257),
+                                 (1, 16), (63, 17), (130, 31), (66, 33), (200, 64),
+                                 (127, 100), (64, 127), (190, 128), (65, 129)])
 def test_random_bit_exact_vs_oracle(ctx, n, m):
     say(f"case: random [{n}][{m}] both forms")
     rng = np.random.default_rng(n * 1000 + m)
