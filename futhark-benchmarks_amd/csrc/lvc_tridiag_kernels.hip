@@ -147,8 +147,9 @@ __global__ __launch_bounds__(kThreads) void tridag_par_kernel(
 //
 // The scan form (tridag_par_staged_kernel) takes chunks of 16 columns
 // aligned to the systems' columns: with its more numerous carries and five
-// divisions a row, 16 is the widest chunk whose in-flight copy (4 x CH
-// values a lane) fits beside the 256 row registers without spilling.
+// divisions a row it is bound by its division chain rather than its reads,
+// and 16 measured best (32, or 32-column chunks aligned to the arrays'
+// lines as Thomas takes them, were slower).
 // Thomas (tridag_seq_aligned_kernel, below) takes 32-column chunks aligned
 // to the arrays' 128-byte lines instead.
 constexpr int kStagedRows = 128;
