@@ -195,6 +195,40 @@ __device__ __forceinline__ void load_seg(const T *__restrict__ p, int i0, int n,
   }
 }
 
+// s_waitcnt operand for vmcnt(0) alone on gfx9 (expcnt 7, lgkmcnt 15 leave
+// those counters unconstrained).
+constexpr int kWaitVmcnt0 = 0x0f70;
+
+// The two halves of load_seg, for callers that make the completeness test
+// once for several segments (so that their 16-byte loads issue together: a
+// test per segment makes each one's loads wait for the ones before).
+template <typename T, int M>
+__device__ __forceinline__ void load_seg_vec(const T *__restrict__ p, int i0, T (&o)[M]) {
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int q = 0; q < M / 4; ++q) {
+      const float4 v = *reinterpret_cast<const float4 *>(p + i0 + 4 * q);
+      o[4 * q] = v.x;
+      o[4 * q + 1] = v.y;
+      o[4 * q + 2] = v.z;
+      o[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < M / 2; ++q) {
+      const double2 v = *reinterpret_cast<const double2 *>(p + i0 + 2 * q);
+      o[2 * q] = v.x;
+      o[2 * q + 1] = v.y;
+    }
+  }
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void load_seg_tail(const T *__restrict__ p, int i0, int n, T (&o)[M]) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) o[k] = (i0 + k < n) ? p[i0 + k] : T(0);
+}
+
 template <typename T, int M>
 __device__ __forceinline__ void store_seg(T *__restrict__ p, int i0, int n,
                                           bool vec, const T (&o)[M]) {
@@ -545,16 +579,32 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
   constexpr bool kClampedLoads = sizeof(T) == 4;
   const int64_t dn = j > 0 ? -(int64_t)nx : 0, up = j < ny - 1 ? (int64_t)nx : 0;
   const int il = min(max(i0 - 1, 0), nx - 1), ir = min(i0 + M, nx - 1);
+  const bool seg_full = vec && i0 + M <= nx;  // this lane's segment is whole
   auto load_rows = [&](int b) {
     const T *row = R + probs[tile.problem + b].state_off + (int64_t)j * nx;
-    load_seg<T, M>(row, i0, nx, vec, rc);
     if constexpr (kClampedLoads) {
-      load_seg<T, M>(row + dn, i0, nx, vec, rm);
-      load_seg<T, M>(row + up, i0, nx, vec, rp);
+      // One test for the three rows, so that their 16-byte loads issue
+      // together.  Both paths write the same registers, so the compiler
+      // would wait for every memory operation in flight (the previous
+      // problem's stores included) before the 16-byte loads, in case the
+      // element loads of lanes past the row end are still landing there;
+      // those lanes wait for their own loads instead, and a wave with none
+      // skips the wait.
+      if (seg_full) {
+        load_seg_vec<T, M>(row, i0, rc);
+        load_seg_vec<T, M>(row + dn, i0, rm);
+        load_seg_vec<T, M>(row + up, i0, rp);
+      } else {
+        load_seg_tail<T, M>(row, i0, nx, rc);
+        load_seg_tail<T, M>(row + dn, i0, nx, rm);
+        load_seg_tail<T, M>(row + up, i0, nx, rp);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      }
       const T vl = row[il], vr = row[ir];
       rl = (i0 > 0 && i0 - 1 < nx) ? vl : T(0);
       rr = (i0 + M < nx) ? vr : T(0);
     } else {
+      load_seg<T, M>(row, i0, nx, vec, rc);
       rl = (i0 > 0 && i0 - 1 < nx) ? row[i0 - 1] : T(0);
       rr = (i0 + M < nx) ? row[i0 + M] : T(0);
       if (j > 0) load_seg<T, M>(row - nx, i0, nx, vec, rm);
@@ -580,8 +630,19 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
     }
   };
   if (NB == 1 || kPrefetch) load_rows(0);
-  load_seg<T, M>(tp.ax, i0, nx, vec, a);
-  load_seg<T, M>(tp.cx, i0, nx, vec, c);
+  if constexpr (kClampedLoads) {
+    if (seg_full) {
+      load_seg_vec<T, M>(tp.ax, i0, a);
+      load_seg_vec<T, M>(tp.cx, i0, c);
+    } else {
+      load_seg_tail<T, M>(tp.ax, i0, nx, a);
+      load_seg_tail<T, M>(tp.cx, i0, nx, c);
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    }
+  } else {
+    load_seg<T, M>(tp.ax, i0, nx, vec, a);
+    load_seg<T, M>(tp.cx, i0, nx, vec, c);
+  }
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     a[k] *= E;
