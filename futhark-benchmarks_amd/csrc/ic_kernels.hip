@@ -601,16 +601,21 @@ __global__ __launch_bounds__(256) void ic_accept_kernel(StepArgs A) {
   const int32_t i = (int32_t)(blockIdx.x * (256 / kSumLanes) + threadIdx.x / kSumLanes);
   float s = 0.0f;
   if (i < pop) {
+    // this lane's terms w = t, t + kSumLanes, ... in that order, eight loads
+    // issued before their sums (a lane has ceil(nswap / kSumLanes) terms: 7
+    // for 196 swaptions, so one batch)
     const float *col = A.terms + i;
-    int32_t w = t;
-    for (; w + 7 * kSumLanes < A.nswap; w += 8 * kSumLanes) {
+    for (int32_t w = t; w < A.nswap; w += 8 * kSumLanes) {
       float v[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) v[k] = col[(int64_t)(w + k * kSumLanes) * pop];
+      for (int k = 0; k < 8; k++) {
+        const int32_t wk = w + k * kSumLanes;
+        v[k] = wk < A.nswap ? col[(int64_t)wk * pop] : 0.0f;
+      }
 #pragma unroll
-      for (int k = 0; k < 8; k++) s += v[k];
+      for (int k = 0; k < 8; k++)
+        if (w + k * kSumLanes < A.nswap) s += v[k];
     }
-    for (; w < A.nswap; w += kSumLanes) s += col[(int64_t)w * pop];
   }
   for (int m = 1; m < kSumLanes; m <<= 1) s += __shfl_xor(s, m, 64);
   if (i >= pop || t != 0) return;
@@ -621,13 +626,18 @@ __global__ __launch_bounds__(256) void ic_accept_kernel(StepArgs A) {
     const float r = sobol(A.dv, A.acc + i);
     if (A.trace.newl) A.trace.newl[(int64_t)A.it * pop + i] = s;
     if (r < acceptance) {
-      for (int j = 0; j < 5; j++) A.genomes[5 * i + j] = A.props[5 * i + j];
+      // genomes and proposals are separate arrays: the five loads first
+      float g[5];
+      for (int j = 0; j < 5; j++) g[j] = A.props[5 * i + j];
+      for (int j = 0; j < 5; j++) A.genomes[5 * i + j] = g[j];
       A.logliks[i] = s;
     }
   }
   if (A.trace.genomes) {
     const int64_t row = (int64_t)(A.it + 1) * pop + i;
-    for (int j = 0; j < 5; j++) A.trace.genomes[5 * row + j] = A.genomes[5 * i + j];
+    float g[5];
+    for (int j = 0; j < 5; j++) g[j] = A.genomes[5 * i + j];
+    for (int j = 0; j < 5; j++) A.trace.genomes[5 * row + j] = g[j];
     A.trace.logliks[row] = A.logliks[i];
   }
 }
