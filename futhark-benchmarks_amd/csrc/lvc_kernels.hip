@@ -1111,7 +1111,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
     for (int q = 0; q < kSegFields; ++q) ftab[6 * LVC_COL_ROWS + q * kColWaves + si] = sv[q];
     // FP64 columns (twice the bytes) start after their loads have landed:
     // measured faster than overlapping the forward sweep with them.
-    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
   }
   const T *F = kFacLds ? ftab : f;
