@@ -1133,13 +1133,17 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
     // exact sum x + e, as an FP64 addition rounded to T would give).
     const double tnow = timeline(pr.t, kt, pr.nt), tnext = timeline(pr.t, kt + 1, pr.nt);
     const double s64 = (double)(T)(1.0 / (tnext - tnow));  // dt as the x sweep rounds it
+    // Every row takes the diagonal s64, also the rows past the grid (whose
+    // system row is the identity): their taps and right-hand side are zero,
+    // so the solve gave them +-0 (NaN only after a non-finite neighbour) and
+    // the residual is the same for any positive diagonal.  No per-row test
+    // (32 row masks held until the stores spilled the scalar registers).
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const double xk = (double)x[k];
       const double xm = (double)(k > 0 ? x[k - 1] : XL);
       const double xp = (double)(k < M - 1 ? x[k + 1] : XN);
-      const double sk = r0 + k < ny ? s64 : 1.0;
-      const double ax = fma(L.c64[r0 + k], xp - xk, fma(L.a64[r0 + k], xm - xk, sk * xk));
+      const double ax = fma(L.c64[r0 + k], xp - xk, fma(L.a64[r0 + k], xm - xk, s64 * xk));
       d[k] = (T)((double)d[k] - ax);
     }
     T EL, EN;
@@ -1148,14 +1152,21 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
     for (int k = 0; k < M; ++k) x[k] += d[k];
   }
   const __amdgpu_buffer_rsrc_t out = grid_rsrc(R + state_off, nx, ny);
+  // With the refinement, the stores' row offsets are formed again here (a
+  // scalar multiply each) rather than shared with the loads': the empty asm
+  // hides that the stride is the loads' one, since 32 offsets held across
+  // the two solves spilled the scalar registers.  (Without it the form
+  // holds no such offsets; the asm would cost vector registers there.)
+  int st_row = row_bytes, st_r0 = r0;
+  if constexpr (REF) asm volatile("" : "+s"(st_row), "+s"(st_r0));
   if (active) {
-    if (r0 + M <= ny) {  // wave-uniform
+    if (st_r0 + M <= ny) {  // wave-uniform
 #pragma unroll
-      for (int k = 0; k < M; ++k) buffer_store<T>(out, x[k], lane_off, (r0 + k) * row_bytes);
+      for (int k = 0; k < M; ++k) buffer_store<T>(out, x[k], lane_off, (st_r0 + k) * st_row);
     } else {
 #pragma unroll
       for (int k = 0; k < M; ++k)
-        if (r0 + k < ny) buffer_store<T>(out, x[k], lane_off, (r0 + k) * row_bytes);
+        if (st_r0 + k < ny) buffer_store<T>(out, x[k], lane_off, (st_r0 + k) * st_row);
     }
   }
   // The last of the problem's workgroups to finish advances its time index
