@@ -1146,8 +1146,13 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
       const double ax = fma(L.c64[r0 + k], xp - xk, fma(L.a64[r0 + k], xm - xk, s64 * xk));
       d[k] = (T)((double)d[k] - ax);
     }
+    // The second solve forms its segment masks again (scalar compares)
+    // rather than holding the first one's across the residual: the empty asm
+    // hides that p, P and gbase are unchanged.
+    int p2 = p, P2 = P, gbase2 = gbase;
+    asm volatile("" : "+s"(p2), "+s"(P2), "+s"(gbase2));
     T EL, EN;
-    col_solve<T, M>(d, L, F, p, P, gbase, lane, 1, EL, EN);
+    col_solve<T, M>(d, L, F, p2, P2, gbase2, lane, 1, EL, EN);
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] += d[k];
   }
