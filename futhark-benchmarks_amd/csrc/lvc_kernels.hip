@@ -505,14 +505,18 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
                                        const T (&s)[M], T (&x)[M]) {
   const T xl = from_below<ROW>(sys.L, x[M - 1], 1, T(0));
   const T xr = from_above<ROW>(sys.L, x[0], 1, T(0));
+  // One difference per pair of neighbours: x[k-1] - x[k] is the negated
+  // x[k] - x[k-1] (round to nearest is symmetric; only the sign of an exact
+  // zero can differ, which no sum here keeps).
   T r[M];
+  double dlo = (double)x[0] - (double)xl;  // x[k] - x[k-1]
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const double xk = (double)x[k];
-    const double xm = (double)(k > 0 ? x[k - 1] : xl);
-    const double xp = (double)(k < M - 1 ? x[k + 1] : xr);
-    const double ax = fma((double)c[k], xp - xk, fma((double)a[k], xm - xk, (double)s[k] * xk));
+    const double dhi = (double)(k < M - 1 ? x[k + 1] : xr) - xk;  // x[k+1] - x[k]
+    const double ax = fma((double)c[k], dhi, fma(-(double)a[k], dlo, (double)s[k] * xk));
     r[k] = (T)((double)d[k] - ax);
+    dlo = dhi;
   }
   T e[M];
   sys.template solve<ROW>(r, e);
@@ -615,14 +619,15 @@ __device__ __forceinline__ void xsweep_tile(const lvc_tile tile,
   // U (the right-hand side) to registers; V (the explicit y term) and, for
   // the refinement, U to LDS.
   auto explicit_terms = [&](T (&U)[M]) {
+    T dlo = rc[0] - rl;  // rc[k] - rc[k-1]; rc[k-1] - rc[k] is its negation (as above)
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const T r0 = rc[k];
-      const T rw = k > 0 ? rc[k - 1] : rl;
-      const T re = k < M - 1 ? rc[k + 1] : rr;
+      const T dhi = (k < M - 1 ? rc[k + 1] : rr) - r0;
       const T dm = j > 0 ? rm[k] - r0 : T(0);
       const T dp = j < ny - 1 ? rp[k] - r0 : T(0);
-      const T ux = fma(c[k], re - r0, a[k] * (rw - r0));  // 0.5 * explicitX
+      const T ux = fma(c[k], dhi, -(a[k] * dlo));  // 0.5 * explicitX
+      dlo = dhi;
       const T vy = T(-2) * fma(cyj, dp, ayj * dm);         // explicitY
       lds_v[k * LVC_BLOCK + threadIdx.x] = vy;
       U[k] = fma(dt, r0, -ux) + vy;
@@ -1138,13 +1143,16 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
     // so the solve gave them +-0 (NaN only after a non-finite neighbour) and
     // the residual is the same for any positive diagonal.  No per-row test
     // (32 row masks held until the stores spilled the scalar registers).
+    // One difference per pair of neighbouring rows (negated for the row
+    // below, as in the x sweep's residual).
+    double dlo = (double)x[0] - (double)XL;
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const double xk = (double)x[k];
-      const double xm = (double)(k > 0 ? x[k - 1] : XL);
-      const double xp = (double)(k < M - 1 ? x[k + 1] : XN);
-      const double ax = fma(L.c64[r0 + k], xp - xk, fma(L.a64[r0 + k], xm - xk, s64 * xk));
+      const double dhi = (double)(k < M - 1 ? x[k + 1] : XN) - xk;
+      const double ax = fma(L.c64[r0 + k], dhi, fma(-L.a64[r0 + k], dlo, s64 * xk));
       d[k] = (T)((double)d[k] - ax);
+      dlo = dhi;
     }
     // The second solve forms its segment masks again (scalar compares)
     // rather than holding the first one's across the residual: the empty asm
