@@ -506,8 +506,8 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
   const T xl = from_below<ROW>(sys.L, x[M - 1], 1, T(0));
   const T xr = from_above<ROW>(sys.L, x[0], 1, T(0));
   // One difference per pair of neighbours: x[k-1] - x[k] is the negated
-  // x[k] - x[k-1] (round to nearest is symmetric; only the sign of an exact
-  // zero can differ, which no sum here keeps).
+  // x[k] - x[k-1] (round to nearest is symmetric: the same value, only an
+  // exact zero may come out with the other sign).
   T r[M];
   double dlo = (double)x[0] - (double)xl;  // x[k] - x[k-1]
 #pragma unroll
