@@ -2,8 +2,8 @@
  * A C caller of the library in the shape of the reference's own callers of
  * the generated API (the vendored lys frontends: shared.c:15-49,
  * console/liblys.c:243-251, console/main.c:176-180, shared.h:20-28):
- * configuration, context, one entry call, values, sync, frees.  It is the
- * program of INTEGRATION.md section 1, compiled and run by the GPU tests.
+ * configuration, context, one entry call, values, sync, frees, compiled
+ * and run by the GPU tests.
  *
  * Usage: lys_caller outer numX numY numT s0 t alpha nu beta
  * Prints the prices one per line; then checks that a bad size fails with a
