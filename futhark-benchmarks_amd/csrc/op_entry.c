@@ -113,7 +113,8 @@ int futhark_entry_main(struct futhark_context *ctx, struct futhark_f32_1d **out0
   const int launch = priced && num_mc_it > 0 && M > 0;
   pthread_mutex_lock(&ctx->lock);
   void *out = NULL, *dv = NULL, *partial = NULL;
-  const size_t dv_bytes = (size_t)(D * U) * OP_DV_STRIDE * sizeof(int32_t);
+  /* the padded direction vectors, then the bridge check's flag */
+  const size_t dv_bytes = (size_t)(D * U) * OP_DV_STRIDE * sizeof(int32_t) + sizeof(int32_t);
   const size_t part_bytes = (size_t)op_blocks(num_mc_it) * (size_t)M * sizeof(double);
   int rc = lvc_dev_alloc(ctx, (size_t)M * sizeof(float), &out);
   if (!rc && launch)

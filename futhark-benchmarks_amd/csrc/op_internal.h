@@ -28,8 +28,9 @@ typedef struct {
  * f64 partial sums it writes are [op_blocks(N)][M]. */
 int op_blocks(int64_t paths);
 
-/* Pads dir_vs ([dims][bits]) into dv ([dims][OP_DV_STRIDE] ints), runs the
- * paths and writes the mean payoff per model to out[M]; every launch on
+/* Pads dir_vs ([dims][bits]) into dv ([dims][OP_DV_STRIDE] ints, then one
+ * more: whether the bridge tables let the paths skip zeroing their rows),
+ * runs the paths and writes the mean payoff per model to out[M]; every launch on
  * `stream` (a hipStream_t).  The contract fixes dims (1, 15 or 1101).
  * Returns 0 or a hipError_t value. */
 int op_launch(void *stream, int contract, const op_args *A, const int32_t *dir_vs,

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
     const float *__restrict__ vols, const float *__restrict__ drifts,
     const float *__restrict__ sts, const float *__restrict__ detvals,
     const float *__restrict__ discts, int bits, int K, int M, int64_t N,
-    double *__restrict__ partial) {
+    const int32_t *__restrict__ bb_ok, double *__restrict__ partial) {
   const Market mk = {cs, vols, drifts, sts, detvals, discts, K};
   constexpr int D = Shape<C>::D, U = Shape<C>::U, DIMS = D * U;
   // The high part of the Sobol numbers: for contract 1's one dimension
@@ -277,6 +277,7 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
   // bits, one multiply instead of an IEEE division
   const float inv_divisor = ldexpf(1.0f, -bits);
   float rows[U][D];
+  const bool zero_free = __builtin_amdgcn_readfirstlane(*bb_ok) != 0;
   // blocks of kThreads Sobol indices covering 0 .. N (index 0 is no path)
   const int64_t nblk = N / kThreads + 1;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
@@ -336,9 +337,14 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
     // The U rows are independent chains: each step reads all of them,
     // then computes, then writes, so a lane overlaps U chains' memory
     // latency (a step's own reads precede its write, as in :157-167).
-    for (int i = 0; i < D; ++i) {
+    // The rows start at zero (:153) only where that can be read: not when
+    // the bridge writes every date once and reads only dates written at an
+    // earlier step (op_bridge_check_kernel), the datasets' tables included.
+    if (!zero_free) {
+      for (int i = 0; i < D; ++i) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) rows[u][i] = 0.0f;
+        for (int u = 0; u < U; ++u) rows[u][i] = 0.0f;
+      }
     }
     {
       const int l0 = clamp(bi[0] - 1);
@@ -378,6 +384,50 @@ __global__ __launch_bounds__(kThreads) void op_paths_kernel(
     if (tid == 0) partial[(int64_t)blockIdx.x * M + m] = red[0];
     __syncthreads();
   }
+}
+
+// Whether the bridge tables make the rows' zero start unobservable: with
+// the path kernel's clamped indexes, step i (of D) writes date l_i and, for
+// i >= 1, reads dates k_i and (when li != 0) j_i.  When every date is written
+// by exactly one step and every read date by an earlier step, each read sees
+// that step's value and every date ends written, with or without the zeros.
+// One workgroup; *ok = 1 when that holds, 0 otherwise or past kCheckDates.
+constexpr int kCheckDates = 4096;
+__global__ __launch_bounds__(kThreads) void op_bridge_check_kernel(const int32_t *__restrict__ bbi,
+                                                                   int D, int32_t *__restrict__ ok) {
+  __shared__ int32_t first[kCheckDates], count[kCheckDates];
+  __shared__ int32_t bad;
+  const int tid = threadIdx.x;
+  if (D > kCheckDates) {
+    if (tid == 0) *ok = 0;
+    return;  // uniform: no barrier passed yet
+  }
+  for (int p = tid; p < D; p += kThreads) {
+    first[p] = 0x7fffffff;
+    count[p] = 0;
+  }
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  const int32_t *bi = bbi, *li = bbi + D, *ri = bbi + 2 * D;
+  auto clamp = [D](int i) { return i < 0 ? 0 : (i >= D ? D - 1 : i); };
+  for (int i = tid; i < D; i += kThreads) {
+    const int l = clamp(bi[i] - 1);
+    atomicMin(&first[l], i);
+    atomicAdd(&count[l], 1);
+  }
+  __syncthreads();
+  int b = 0;
+  for (int i = tid; i < D; i += kThreads) {
+    if (count[i] != 1) b = 1;
+    if (i >= 1) {
+      if (first[clamp(ri[i] - 1)] >= i) b = 1;
+      const int j = li[i] - 1;
+      if (j != -1 && first[clamp(j)] >= i) b = 1;
+    }
+  }
+  if (b) atomicOr(&bad, 1);
+  __syncthreads();
+  if (tid == 0) *ok = bad ? 0 : 1;
 }
 
 // The mean per model: the workgroups' partial sums by a fixed tree, / N.
@@ -426,6 +476,8 @@ int op_launch(void *stream, int contract, const op_args *A, const int32_t *dir_v
   op_pad_kernel<<<(np + 255) / 256, 256, 0, st>>>(dir_vs, dv, dims, A->bits);
   const int nblocks = op_blocks(A->N);
   const int D = contract == 1 ? 1 : contract == 2 ? 5 : 367, U = contract == 1 ? 1 : 3;
+  int32_t *bb_ok = dv + np;  // the bridge check's flag, after the padded vectors
+  op_bridge_check_kernel<<<1, kThreads, 0, st>>>(A->bbi, D, bb_ok);
   // the models in chunks of OP_MAX_MODELS (a lane's sums are registers): each
   // chunk reruns the paths with its models' market data and writes its means
   for (int m0 = 0; m0 < A->M; m0 += OP_MAX_MODELS) {
@@ -433,7 +485,7 @@ int op_launch(void *stream, int contract, const op_args *A, const int32_t *dir_v
 #define OP_PATHS_ARGS                                                                   \
   dv, A->bbi, A->bbd, A->cs + (int64_t)m0 * U * U, A->vols + (int64_t)m0 * D * U,       \
       A->drifts + (int64_t)m0 * D * U, A->sts + (int64_t)m0 * U, A->detvals + m0,       \
-      A->discts + (int64_t)m0 * A->K, A->bits, A->K, mc, A->N, partial
+      A->discts + (int64_t)m0 * A->K, A->bits, A->K, mc, A->N, bb_ok, partial
     switch (contract) {
       case 1: op_paths_kernel<1><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
       case 2: op_paths_kernel<2><<<nblocks, kThreads, 0, st>>>(OP_PATHS_ARGS); break;
