@@ -129,17 +129,21 @@ def test_deterministic_paths_closed_form(ctx, contract):
     np.testing.assert_allclose(got, oracle.op_main(*args), rtol=RTOL)
 
 
-def _random_bridge(D, rng, permutation=True, scale=0.1):
+def _random_bridge(D, rng, permutation=True, scale=0.1, late_reads=False):
     """bb_inds / bb_data for a bridge in a random order: step i writes date
     bi[i] from an earlier-written right neighbour and, or not (li = 0, j =
     -1), an earlier-written left one.  Without `permutation` some dates are
-    written twice and others never (they stay 0, :153)."""
+    written twice and others never (they stay 0, :153).  With `late_reads`
+    every third step's right neighbour is a date written at a later step,
+    which the reference reads as the initial 0."""
     bi = rng.permutation(D) + 1 if permutation else rng.integers(1, D + 1, D)
     li = np.zeros(D, np.int64)
     ri = np.zeros(D, np.int64)
     for i in range(1, D):
         ri[i] = bi[rng.integers(0, i)]
         li[i] = 0 if rng.random() < 0.3 else bi[rng.integers(0, i)]
+        if late_reads and i % 3 == 1 and i + 1 < D:
+            ri[i] = bi[rng.integers(i + 1, D)]
     ri[0] = li[0] = 1
     inds = np.stack([bi, li, ri]).astype(np.int32)
     data = (rng.random((3, D)) * np.array([[scale], [0.5], [0.5]])).astype(np.float32)
@@ -147,14 +151,17 @@ def _random_bridge(D, rng, permutation=True, scale=0.1):
 
 
 @pytest.mark.parametrize("name", ["medium", "large"])
-@pytest.mark.parametrize("permutation", [True, False])
-def test_random_bridge_tables(ctx, datasets, name, permutation):
-    """The generic bridge indexing (:153-177) beyond the datasets' tables."""
+@pytest.mark.parametrize("permutation,late_reads", [(True, False), (False, False), (True, True)])
+def test_random_bridge_tables(ctx, datasets, name, permutation, late_reads):
+    """The generic bridge indexing (:153-177) beyond the datasets' tables:
+    a valid bridge (the kernel skips the rows' zero fill), dates written
+    twice or never, and reads of dates written later (both keep it)."""
     args, _ = datasets[name]
-    rng = np.random.default_rng(11 if permutation else 12)
+    rng = np.random.default_rng(13 if late_reads else 11 if permutation else 12)
     D = args[9].shape[1]
     # large: wide enough paths that the barriers of payoff3 are reached
-    bbi, bbd = _random_bridge(D, rng, permutation, scale=0.1 if name == "medium" else 4.0)
+    bbi, bbd = _random_bridge(D, rng, permutation, scale=0.1 if name == "medium" else 4.0,
+                              late_reads=late_reads)
     a = args[:9] + (bbi, bbd)
     n = 3000
     got = ctx.main(a[0], n, *a[2:])
