@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *_
   T Xm = T(0), Xc = T(0), Xp = T(0), nxt = T(0);
 #pragma unroll
   for (int q = kColWaves - 1; q >= 0; --q) {
-    if (q < P) {
+    if (q < P && q >= p - 1) {  // wave-uniform: the sweep stops at X_{p-1}
       nxt = fma(-F[6 * LVC_COL_ROWS + kSrC * kColWaves + q], nxt, Yr[q]) * F[6 * LVC_COL_ROWS + kSrrb * kColWaves + q];
       if (q == p + 1) Xp = nxt;
       if (q == p) Xc = nxt;
