@@ -508,13 +508,18 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
   // One difference per pair of neighbours: x[k-1] - x[k] is the negated
   // x[k] - x[k-1] (round to nearest is symmetric: the same value, only an
   // exact zero may come out with the other sign).
+  // One diagonal for every row: s[0] (dt) on a lane with any row of the
+  // system, whose rows come first; rows past its end (diagonal 1) have zero
+  // taps and right-hand side, so x there is +-0 (or NaN/inf after a
+  // non-finite neighbour) and s * x has the same value for any positive s.
+  const double s64 = (double)s[0];
   T r[M];
   double dlo = (double)x[0] - (double)xl;  // x[k] - x[k-1]
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const double xk = (double)x[k];
     const double dhi = (double)(k < M - 1 ? x[k + 1] : xr) - xk;  // x[k+1] - x[k]
-    const double ax = fma((double)c[k], dhi, fma(-(double)a[k], dlo, (double)s[k] * xk));
+    const double ax = fma((double)c[k], dhi, fma(-(double)a[k], dlo, s64 * xk));
     r[k] = (T)((double)d[k] - ax);
     dlo = dhi;
   }
