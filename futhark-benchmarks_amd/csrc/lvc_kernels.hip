@@ -505,15 +505,15 @@ __device__ __forceinline__ void refine(const SegSys<T, M> &sys, const T (&d)[M],
                                        const T (&s)[M], T (&x)[M]) {
   const T xl = from_below<ROW>(sys.L, x[M - 1], 1, T(0));
   const T xr = from_above<ROW>(sys.L, x[0], 1, T(0));
-  // One difference per pair of neighbours: x[k-1] - x[k] is the negated
-  // x[k] - x[k-1] (round to nearest is symmetric: the same value, only an
-  // exact zero may come out with the other sign).
   // One diagonal for every row: s[0] (dt) on a lane with any row of the
   // system, whose rows come first; rows past its end (diagonal 1) have zero
   // taps and right-hand side, so x there is +-0 (or NaN/inf after a
   // non-finite neighbour) and s * x has the same value for any positive s.
   const double s64 = (double)s[0];
   T r[M];
+  // One difference per pair of neighbours: x[k-1] - x[k] is the negated
+  // x[k] - x[k-1] (round to nearest is symmetric: the same value, only an
+  // exact zero may come out with the other sign).
   double dlo = (double)x[0] - (double)xl;  // x[k] - x[k-1]
 #pragma unroll
   for (int k = 0; k < M; ++k) {
