@@ -330,6 +330,17 @@ def op_bridge(bb_inds: np.ndarray, bb_data: np.ndarray, gauss: np.ndarray) -> np
     return out
 
 
+def _threads(nthreads=None) -> int:
+    """Worker threads for the oracle's OpenMP loops: the caller's count, else
+    OMP_NUM_THREADS, else 16; never more than the host's CPUs (a shared GPU
+    host's share is smaller than its core count)."""
+    if nthreads:
+        return int(nthreads)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    t = int(env) if env.isdigit() and int(env) > 0 else 16
+    return max(1, min(t, os.cpu_count() or 1))
+
+
 def op_main(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, md_detvals,
             md_discts, bb_inds, bb_data, paths=None, nthreads=None) -> np.ndarray:
     """main (:295-322) over the first `paths` paths (default num_mc_it): the
@@ -350,7 +361,7 @@ def op_main(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, md_d
     L.op_sum_payoffs.restype = ctypes.c_int
     rc = L.op_sum_payoffs(int(contract), 0, n, _p(dv), dv.shape[1], _p(cs), _p(vols),
                           _p(drifts), _p(sts), _p(detv), _p(disc), K, _p(bbi), _p(bbd),
-                          D, U, M, _p(sums), int(nthreads or os.cpu_count() or 1))
+                          D, U, M, _p(sums), _threads(nthreads))
     if rc != 0:
         raise ValueError(f"contract {contract} does not take [{D}][{U}] paths")
     return (sums / n).astype(np.float32)
@@ -374,7 +385,7 @@ def op_payoffs(contract, num_mc_it, dir_vs, md_cs, md_vols, md_drifts, md_sts, m
     L.op_payoffs.restype = ctypes.c_int
     if L.op_payoffs(int(contract), lo, hi, _p(dv), dv.shape[1], _p(cs), _p(vols), _p(drifts),
                     _p(sts), _p(detv), _p(disc), K, _p(bbi), _p(bbd), D, U, M, _p(out),
-                    int(nthreads or os.cpu_count() or 1)) != 0:
+                    _threads(nthreads)) != 0:
         raise ValueError(f"contract {contract} does not take [{D}][{U}] paths")
     return out
 
@@ -503,7 +514,7 @@ def ic_logliks(genomes, swaptions, herm_x, herm_w, nthreads=None) -> np.ndarray:
     _ic_sig("ic_logliks", None, [ctypes.c_int, _ptr, ctypes.c_int, _ptr, ctypes.c_int, _ptr,
                                  _ptr, _ptr, ctypes.c_int])(
         len(g), _p(g), len(sw), _p(sw), len(hx), _p(hx), _p(hw), _p(out),
-        int(nthreads or os.cpu_count() or 1))
+        _threads(nthreads))
     return out
 
 
@@ -556,7 +567,7 @@ def ic_main(pop, mcmc_conv, swaptions, herm_x, herm_w, dir_vct, nthreads=None) -
                  [_c_i32, _c_i32, _c_i32, _ptr, _c_i32, _ptr, _ptr, _c_i32, _ptr, _ptr, _ptr,
                   _ptr, _ptr, ctypes.c_int])(
         int(pop), int(mcmc_conv), len(sw), _p(sw), len(hx), _p(hx), _p(hw), len(dv), _p(dv),
-        _p(out6), _p(summ), _p(fg), _p(fl), int(nthreads or os.cpu_count() or 1))
+        _p(out6), _p(summ), _p(fg), _p(fl), _threads(nthreads))
     if rc != 0:
         raise ValueError("InterestCalib main: the reference fails on these arguments")
     return out6, summ, fg, fl

@@ -170,6 +170,25 @@ def test_random_bridge_tables(ctx, datasets, name, permutation, late_reads):
     np.testing.assert_allclose(got, want, rtol=RTOL)
 
 
+@pytest.mark.parametrize("late_reads", [False, True])
+def test_bridge_tables_two_paths_per_lane(ctx, datasets, late_reads):
+    """Past 2048 x 256 paths the path kernel's lanes run a second path, whose
+    bridge rows start with the first path's values: a valid bridge (the
+    rows' zero fill skipped) must write every date before reading it, and a
+    table that reads dates written later must keep the fill (those reads
+    see 0, :153)."""
+    args, _ = datasets["large"]
+    rng = np.random.default_rng(21 if late_reads else 22)
+    D = args[9].shape[1]
+    bbi, bbd = _random_bridge(D, rng, True, scale=4.0, late_reads=late_reads)
+    a = args[:9] + (bbi, bbd)
+    n = 2048 * 256 + 5000
+    got = ctx.main(a[0], n, *a[2:])
+    want = oracle.op_main(a[0], n, *a[2:])
+    assert np.all(np.isfinite(want))
+    np.testing.assert_allclose(got, want, rtol=RTOL)
+
+
 def test_profiling_report(datasets):
     """With profiling on, futhark_context_report counts the entry's kernel
     launches and their time (JSON, as the generated API's report)."""

@@ -40,19 +40,19 @@ def _main_subset(inp, idx, dtype, thomas=False):
     strikes = (real(0.001) * np.asarray(idx).astype(real)).astype(real)
     return oracle.batch([nx] * n, [ny] * n, [nt] * n, [s0] * n, strikes, [t] * n,
                         [alpha] * n, [nu] * n, [beta] * n, dtype=dtype,
-                        thomas=thomas, nthreads=os.cpu_count() or 1)
+                        thomas=thomas, nthreads=oracle._threads())
 
 
 def check_fixture_small():
     inp, exp = load_dataset("small")
-    got = oracle.main(*inp, dtype=np.float32, nthreads=os.cpu_count() or 1)
+    got = oracle.main(*inp, dtype=np.float32, nthreads=oracle._threads())
     assert got.shape == exp.shape
     np.testing.assert_allclose(got, exp, rtol=0, atol=F32_FIXTURE_ABS)
 
 
 def check_fixture_medium_f64():
     inp, exp = load_dataset("medium")
-    got = oracle.main(*inp, dtype=np.float64, nthreads=os.cpu_count() or 1)
+    got = oracle.main(*inp, dtype=np.float64, nthreads=oracle._threads())
     np.testing.assert_allclose(got, exp, rtol=0, atol=F64_MEDIUM_ABS)
 
 
@@ -64,7 +64,7 @@ def test_fixture_small_f32():
 
 def test_fixture_medium_f32():
     inp, exp = load_dataset("medium")
-    got = oracle.main(*inp, dtype=np.float32, nthreads=os.cpu_count() or 1)
+    got = oracle.main(*inp, dtype=np.float32, nthreads=oracle._threads())
     np.testing.assert_allclose(got, exp, rtol=0, atol=F32_FIXTURE_ABS)
 
 
@@ -258,7 +258,7 @@ def test_grid32_vs_f32_restatement(nx):
         sets.append(_f32(rng.uniform(0.02, 0.05), rng.uniform(1, 10), rng.uniform(0.1, 0.3),
                          rng.uniform(0.4, 0.8), rng.uniform(0.3, 0.7)))
     tol = grid32_tolerance(nx)
-    nth = os.cpu_count() or 1
+    nth = oracle._threads()
     for i, (s0, t, a, nu, b) in enumerate(sets):
         p32 = oracle.main(6, nx, 32, 12, *map(np.float32, (s0, t, a, nu, b)),
                           dtype=np.float32, nthreads=nth)
