@@ -984,7 +984,7 @@ struct ColLds {
 // Solves in place for the right-hand side held in x (this lane's segment p of
 // one column), using exchange slot `sl`.  Returns the neighbouring unknowns
 // XL = x_{-1} (previous segment's last) and XN = x_M (next segment's first).
-template <typename T, int M>
+template <typename T, int M, bool HOIST>
 __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *__restrict__ F, int p, int P,
                                           int gbase, int lane, int sl, T &XL, T &XN) {
   const int r0 = p * M;
@@ -998,6 +998,28 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *_
   const int me = (gbase + p) * 64 + lane;
   L.y0[sl][me] = x[0];
   L.Dt[sl][me] = fma(-F[6 * LVC_COL_ROWS + kSaL * kColWaves + p], x[M - 2], dlast);
+  // HOIST (a factor table in global memory, read by scalar loads): the
+  // reduced system's factors of every segment are read here, one round trip
+  // that the barrier hides, rather than one read and wait per segment inside
+  // the serial sweeps below.  From LDS they are read where used (held in
+  // vector registers they would spill the FP64 form).
+  const T *Fs = F + 6 * LVC_COL_ROWS;
+  T rcL[kColWaves], rm[kColWaves], rC[kColWaves], rrb[kColWaves];
+  if constexpr (HOIST) {
+#pragma unroll
+    for (int q = 0; q < kColWaves; ++q) {
+      rcL[q] = Fs[kScL * kColWaves + q];
+      rm[q] = Fs[kSrm * kColWaves + q];
+      rC[q] = Fs[kSrC * kColWaves + q];
+      rrb[q] = Fs[kSrrb * kColWaves + q];
+    }
+  }
+  auto seg = [&](const T(&h)[kColWaves], int which, int q) {
+    if constexpr (HOIST)
+      return h[q];
+    else
+      return Fs[which * kColWaves + q];
+  };
   __syncthreads();
   // Reduced system of the column: Thomas over its P segments (slots
   // gbase .. gbase+P-1, all below kColWaves).
@@ -1009,8 +1031,8 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *_
       const int at = (gbase + q) * 64 + lane;
       const T yn = q + 1 < P ? L.y0[sl][at + 64] : T(0);
       if (q == p) y0n = yn;
-      const T D = fma(-F[6 * LVC_COL_ROWS + kScL * kColWaves + q], yn, L.Dt[sl][at]);
-      prev = q == 0 ? D : fma(-F[6 * LVC_COL_ROWS + kSrm * kColWaves + q], prev, D);
+      const T D = fma(-seg(rcL, kScL, q), yn, L.Dt[sl][at]);
+      prev = q == 0 ? D : fma(-seg(rm, kSrm, q), prev, D);
       Yr[q] = prev;
     }
   }
@@ -1018,7 +1040,7 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *_
 #pragma unroll
   for (int q = kColWaves - 1; q >= 0; --q) {
     if (q < P && q >= p - 1) {  // wave-uniform: the sweep stops at X_{p-1}
-      nxt = fma(-F[6 * LVC_COL_ROWS + kSrC * kColWaves + q], nxt, Yr[q]) * F[6 * LVC_COL_ROWS + kSrrb * kColWaves + q];
+      nxt = fma(-seg(rC, kSrC, q), nxt, Yr[q]) * seg(rrb, kSrrb, q);
       if (q == p + 1) Xp = nxt;
       if (q == p) Xc = nxt;
       if (q == p - 1) Xm = nxt;
@@ -1136,7 +1158,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
 #pragma unroll
   for (int k = 0; k < M; ++k) x[k] = d[k];
   T XL, XN;
-  col_solve<T, M>(x, L, F, p, P, gbase, lane, 0, XL, XN);
+  col_solve<T, M, REF>(x, L, F, p, P, gbase, lane, 0, XL, XN);
   if (REF) {
     // Residual in FP64 (T-rounded taps, as above), solved again with the same
     // factorisation; the correction is added in T (one rounding of the
@@ -1165,7 +1187,7 @@ __global__ __launch_bounds__(LVC_COL_BLOCK, M <= 32 ? 4 : 2) void lvc_ycol_kerne
     int p2 = p, P2 = P, gbase2 = gbase;
     asm volatile("" : "+s"(p2), "+s"(P2), "+s"(gbase2));
     T EL, EN;
-    col_solve<T, M>(d, L, F, p2, P2, gbase2, lane, 1, EL, EN);
+    col_solve<T, M, REF>(d, L, F, p2, P2, gbase2, lane, 1, EL, EN);
 #pragma unroll
     for (int k = 0; k < M; ++k) x[k] += d[k];
   }
