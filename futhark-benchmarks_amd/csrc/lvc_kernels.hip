@@ -1022,16 +1022,24 @@ __device__ __forceinline__ void col_solve(T (&x)[M], ColLds<T, M> &L, const T *_
   };
   __syncthreads();
   // Reduced system of the column: Thomas over its P segments (slots
-  // gbase .. gbase+P-1, all below kColWaves).
+  // gbase .. gbase+P-1, all below kColWaves).  Its LDS operands are read
+  // first, all at once (the slot index clamped inside the workgroup's
+  // slots; values past the column's segments are not used), so that the
+  // serial sweep does not wait on one LDS round trip per segment.
+  T ys[kColWaves], ds[kColWaves];
+#pragma unroll
+  for (int q = 0; q < kColWaves; ++q) {
+    ys[q] = L.y0[sl][min(gbase + q + 1, kColWaves - 1) * 64 + lane];
+    ds[q] = L.Dt[sl][min(gbase + q, kColWaves - 1) * 64 + lane];
+  }
   T Yr[kColWaves], y0n = T(0);
   T prev = T(0);
 #pragma unroll
   for (int q = 0; q < kColWaves; ++q) {
     if (q < P) {
-      const int at = (gbase + q) * 64 + lane;
-      const T yn = q + 1 < P ? L.y0[sl][at + 64] : T(0);
+      const T yn = q + 1 < P ? ys[q] : T(0);
       if (q == p) y0n = yn;
-      const T D = fma(-seg(rcL, kScL, q), yn, L.Dt[sl][at]);
+      const T D = fma(-seg(rcL, kScL, q), yn, ds[q]);
       prev = q == 0 ? D : fma(-seg(rm, kSrm, q), prev, D);
       Yr[q] = prev;
     }
